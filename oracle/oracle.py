@@ -1,0 +1,202 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and of the reference harness.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` as the checker; the product path
+(``staggered-grid_amd/``) never imports this module.
+
+The oracle restates ``3DCPUwithPML_3DArray_GridIncludePMLandOrder`` (see
+``sg_oracle.h`` for the line-by-line citations).  Arrays cross this boundary in
+the solver's x-fastest layout, shape ``(nz, ny, nx)`` in numpy terms.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+import subprocess
+import tempfile
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+DAMP_REFERENCE = 0
+DAMP_QUADRATIC = 1
+INJECT_DOUBLE = 0
+INJECT_FLOAT = 1
+RICKER_CEIL_DELAY = 0
+RICKER_NO_DELAY = 1
+
+FIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+# Coefficient literals of DIFF_10 (3DCPUwithPML.../ElasticMacro.h:65-69), also
+# DifferenceCoefficient in 3DGPUwithPML_1DArraySingleGPU_concise/main.cu:105-109.
+REF_COEF = (1.21124, -8.97217e-2, 1.38428e-2, -1.76566e-3, 1.18680e-4)
+# Exact 10th-order staggered coefficients (closed form).
+EXACT_COEF = (19845 / 16384, -735 / 8192, 567 / 40960, -405 / 229376, 35 / 294912)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("pml", ctypes.c_int),
+        ("dt", ctypes.c_float), ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float),
+        ("coef", ctypes.c_float * 5),
+        ("nt", ctypes.c_int),
+        ("src", ctypes.c_int * 3),
+        ("damp_mode", ctypes.c_int),
+        ("damp_v", ctypes.c_float),
+        ("damp_r", ctypes.c_float),
+        ("damp_tau", ctypes.c_float),
+        ("inject_mode", ctypes.c_int),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+@dataclass
+class OracleParams:
+    nx: int
+    ny: int
+    nz: int
+    pml: int
+    dt: float
+    dx: float
+    dy: float
+    dz: float
+    nt: int
+    src: tuple
+    coef: tuple = REF_COEF
+    damp_mode: int = DAMP_REFERENCE
+    damp_v: float = 2000.0
+    damp_r: float = 0.001
+    damp_tau: float = 3.4
+    inject_mode: int = INJECT_DOUBLE
+    nthreads: int = field(default_factory=lambda: max(1, min(8, os.cpu_count() or 1)))
+
+    def to_c(self) -> _Params:
+        p = _Params()
+        p.nx, p.ny, p.nz, p.pml = self.nx, self.ny, self.nz, self.pml
+        p.dt, p.dx, p.dy, p.dz = self.dt, self.dx, self.dy, self.dz
+        for i, c in enumerate(self.coef):
+            p.coef[i] = c
+        p.nt = self.nt
+        for i in range(3):
+            p.src[i] = int(self.src[i])
+        p.damp_mode, p.damp_v, p.damp_r, p.damp_tau = self.damp_mode, self.damp_v, self.damp_r, self.damp_tau
+        p.inject_mode, p.nthreads = self.inject_mode, self.nthreads
+        return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_ricker.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, fp]
+        L.oracle_ricker.restype = None
+        L.oracle_damping_profile_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                                 ctypes.c_float, ctypes.c_float, ctypes.c_float, fp]
+        L.oracle_damping_profile_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.oracle_run_f32.argtypes = [ctypes.POINTER(_Params), fp, fp, fp, fp, ip, ctypes.c_int, ctypes.c_uint,
+                                     fp, fp]
+        L.oracle_run_f32.restype = ctypes.c_int
+        L.oracle_run_f64.argtypes = [ctypes.POINTER(_Params), dp, dp, dp, dp, ip, ctypes.c_int, ctypes.c_uint,
+                                     dp, dp]
+        L.oracle_run_f64.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct)) if a is not None else None
+
+
+def ricker(f0: float, dt: float, nt: int, delay_mode: int = RICKER_CEIL_DELAY) -> np.ndarray:
+    """s(it) of sourceFunction (ElasticWaveEquation3D.cpp:114-119), float arithmetic."""
+    out = np.zeros(nt, np.float32)
+    lib().oracle_ricker(f0, dt, nt, delay_mode, _ptr(out, ctypes.c_float))
+    return out
+
+
+def damping_profile(n, pml, delta, mode=DAMP_REFERENCE, v=2000.0, r=0.001, tau=3.4, dtype=np.float32):
+    out = np.zeros(n, dtype)
+    if dtype == np.float32:
+        lib().oracle_damping_profile_f32(n, pml, delta, mode, v, r, tau, _ptr(out, ctypes.c_float))
+    else:
+        lib().oracle_damping_profile_f64(n, pml, delta, mode, v, r, tau, _ptr(out, ctypes.c_double))
+    return out
+
+
+def run(p: OracleParams, b, lam, mu, wavelet, stations=None, comp_mask: int = 0x1FF,
+        fields: bool = True, dtype=np.float32):
+    """Run the oracle.  Returns (traces[nt, nst, ncomp], fields[9, nz, ny, nx] or None)."""
+    ct = ctypes.c_float if dtype == np.float32 else ctypes.c_double
+    shape = (p.nz, p.ny, p.nx)
+    b = np.ascontiguousarray(b, dtype).reshape(shape)
+    lam = np.ascontiguousarray(lam, dtype).reshape(shape)
+    mu = np.ascontiguousarray(mu, dtype).reshape(shape)
+    wavelet = np.ascontiguousarray(wavelet, dtype)
+    assert wavelet.size >= p.nt
+    st = np.ascontiguousarray(stations if stations is not None else np.zeros((0, 3)), np.int32).reshape(-1, 3)
+    ncomp = bin(comp_mask & 0x1FF).count("1")
+    tr = np.zeros((p.nt, st.shape[0], ncomp), dtype)
+    fo = np.zeros((9,) + shape, dtype) if fields else None
+    fn = lib().oracle_run_f32 if dtype == np.float32 else lib().oracle_run_f64
+    rc = fn(ctypes.byref(p.to_c()), _ptr(b, ct), _ptr(lam, ct), _ptr(mu, ct), _ptr(wavelet, ct),
+            _ptr(st, ctypes.c_int), st.shape[0], comp_mask, _ptr(tr, ct), _ptr(fo, ct) if fields else None)
+    if rc != 0:
+        raise ValueError(f"oracle_run failed with status {rc}")
+    return tr, fo
+
+
+# ---------------------------------------------------------------------------
+# Reference harness (oracle/_ref, built from the reference sources in this
+# container by `make -C oracle ref`).  Used to write golden vectors.
+# ---------------------------------------------------------------------------
+
+def ref_available(kind: str = "native") -> bool:
+    return os.path.exists(os.path.join(REF_DIR, f"ref_{kind}"))
+
+
+def ref_run(kind: str, interior, pml, nt, src, stations, dx, dy, dz, dt, f0, damp_v, b, lam, mu):
+    """Run the unmodified reference CPU solver through the harness.
+
+    ``interior`` is the reference GRID (interior size); model arrays are
+    x-fastest over the total grid.  Returns dict(wavelet, traces[nt,nst,9],
+    fields[9,nz,ny,nx], damping=(dx_line, dy_line, dz_line)).
+    """
+    exe = os.path.join(REF_DIR, f"ref_{kind}")
+    nxi, nyi, nzi = interior
+    NX, NY, NZ = nxi + 2 * pml + 10, nyi + 2 * pml + 10, nzi + 2 * pml + 10
+    N = NX * NY * NZ
+    st = np.ascontiguousarray(stations, np.int32).reshape(-1, 3)
+    with tempfile.TemporaryDirectory(dir=REF_DIR) as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<9i", nxi, nyi, nzi, pml, nt, *[int(s) for s in src], st.shape[0]))
+            f.write(struct.pack("<6f", dx, dy, dz, dt, f0, damp_v))
+            f.write(st.tobytes())
+            for a in (b, lam, mu):
+                f.write(np.ascontiguousarray(a, np.float32).reshape(-1).tobytes())
+        subprocess.run([exe, fin, fout, td], check=True)
+        raw = np.fromfile(fout, np.float32)
+    o = 0
+    wavelet = raw[o:o + nt]; o += nt
+    traces = raw[o:o + nt * st.shape[0] * 9].reshape(nt, st.shape[0], 9); o += nt * st.shape[0] * 9
+    fields = raw[o:o + 9 * N].reshape(9, NZ, NY, NX); o += 9 * N
+    dxl = raw[o:o + NX]; o += NX
+    dyl = raw[o:o + NY]; o += NY
+    dzl = raw[o:o + NZ]; o += NZ
+    assert o == raw.size
+    return dict(wavelet=wavelet, traces=traces, fields=fields, damping=(dxl, dyl, dzl))

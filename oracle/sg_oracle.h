@@ -1,0 +1,97 @@
+/*
+ * sg_oracle.h -- CPU restatement of the reference split-field PML solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP solver, not part
+ * of the product: only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it.  Nothing in staggered-grid_amd/ links or calls it.
+ *
+ * What it restates (paths relative to the reference root):
+ *   3DCPUwithPML_3DArray_GridIncludePMLandOrder/ElasticWaveEquation3D.cpp
+ *     propagate        :53-85   (step order: source, velocity, stress, record)
+ *     sourceFunction   :114-119 (Ricker, float arithmetic, ceil() delay)
+ *     loadSeismicSource:121-130 (sxx/syy/szz += pow(dt,2)*s, double add)
+ *     velocityUpdate   :134-183 (split form everywhere, 9 splits)
+ *     stressUpdate     :185-249 (split form everywhere, 18 splits)
+ *     calculateDamping :253-300 (damping only outside the interior box)
+ *   .../ElasticMacro.h:16-23 (DAMPING0/DAMPING), :63-91 (DIFF_10),
+ *                     :101-114 (BUOYANCYBAR*, MUBAR*)
+ *
+ * Layout: the reference keeps float*** arrays indexed [i][j][k] with k
+ * contiguous.  The oracle keeps that internal layout and loop nest (i, j, k
+ * with k innermost); its inputs and outputs are converted from/to the
+ * solver's x-fastest layout (index = i + j*nx + k*nx*ny) at the boundary.
+ *
+ * Grid convention: nx, ny, nz are TOTAL extents (the reference CPU class takes
+ * the interior size and allocates interior + 2*pml + ORDER; total = that sum).
+ * Updated points are [5, n-5) on each axis; the 5-cell halo is never written.
+ */
+#ifndef SG_ORACLE_H
+#define SG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORACLE_DAMP_REFERENCE = 0, /* log10(1/R)*tau*V/(P*delta) * (dist/P)^2, float (ElasticMacro.h:21-23) */
+    ORACLE_DAMP_QUADRATIC = 1  /* -3*V*ln(R)/(2*P*delta) * (dist/P)^2, d0 in double then float */
+};
+
+enum {
+    ORACLE_INJECT_DOUBLE = 0,  /* sigma += pow(dt,2)*s in double, stored float (ElasticWaveEquation3D.cpp:123-125) */
+    ORACLE_INJECT_FLOAT = 1    /* sigma += dt*dt*s in float (GPU variant, ..._concise/ElasticWaveEquation.cu:216-218) */
+};
+
+enum {
+    ORACLE_RICKER_CEIL_DELAY = 0, /* t0 = ceil(1/(f0*dt)) steps, peak near 2/f0 (ElasticWaveEquation3D.cpp:116) */
+    ORACLE_RICKER_NO_DELAY = 1    /* t0 = 0, peak at 1/f0 */
+};
+
+typedef struct oracle_params {
+    int nx, ny, nz;     /* total extents incl. PML and the 5-cell halo */
+    int pml;            /* PML thickness in cells */
+    float dt, dx, dy, dz;
+    float coef[5];      /* signed staggered coefficients c1..c5 */
+    int nt;             /* number of time steps */
+    int src[3];         /* absolute source index (x, y, z) */
+    int damp_mode;      /* ORACLE_DAMP_* */
+    float damp_v;       /* V in the damping formula */
+    float damp_r;       /* R (target reflection), 0.001 in the reference */
+    float damp_tau;     /* tau, 3.4 in the reference CPU path */
+    int inject_mode;    /* ORACLE_INJECT_* */
+    int nthreads;       /* OpenMP threads (results do not depend on it) */
+} oracle_params;
+
+/* Station component bits (order of the recorded components). */
+enum { OR_VX = 0, OR_VY, OR_VZ, OR_SXX, OR_SYY, OR_SZZ, OR_SXY, OR_SXZ, OR_SYZ, OR_NFIELD };
+
+/* Ricker wavelet s(it), it = 0..nt-1, in the reference's float arithmetic. */
+void oracle_ricker(float f0, float dt, int nt, int delay_mode, float *s_out);
+
+/* 1-D damping profile d(i), i = 0..n-1 for one axis of total extent n. */
+void oracle_damping_profile_f32(int n, int pml, float delta, int mode, float v, float r, float tau,
+                                float *out);
+void oracle_damping_profile_f64(int n, int pml, double delta, int mode, double v, double r, double tau,
+                                double *out);
+
+/*
+ * Run the full solve.  Model arrays are x-fastest over the total grid.
+ * wavelet[it] is s(it) (float); the injection follows p->inject_mode.
+ * stations: nst triples (x, y, z); comp_mask: bits of OR_*; st_out receives
+ * nt x nst x popcount(comp_mask) values (step-major), may be NULL.
+ * fields_out: 9 x (nx*ny*nz) x-fastest in OR_* order, may be NULL.
+ * Returns 0 on success, negative on invalid input or allocation failure.
+ */
+int oracle_run_f32(const oracle_params *p, const float *b, const float *lam, const float *mu,
+                   const float *wavelet, const int *stations, int nst, unsigned comp_mask,
+                   float *st_out, float *fields_out);
+
+/* The same algorithm in double precision (brute-force yardstick). */
+int oracle_run_f64(const oracle_params *p, const double *b, const double *lam, const double *mu,
+                   const double *wavelet, const int *stations, int nst, unsigned comp_mask,
+                   double *st_out, double *fields_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
