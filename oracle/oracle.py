@@ -27,6 +27,7 @@ DAMP_REFERENCE = 0
 DAMP_QUADRATIC = 1
 INJECT_DOUBLE = 0
 INJECT_FLOAT = 1
+INJECT_ACCUMULATE = 2
 RICKER_CEIL_DELAY = 0
 RICKER_NO_DELAY = 1
 

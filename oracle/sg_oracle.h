@@ -39,7 +39,14 @@ enum {
 
 enum {
     ORACLE_INJECT_DOUBLE = 0,  /* sigma += pow(dt,2)*s in double, stored float (ElasticWaveEquation3D.cpp:123-125) */
-    ORACLE_INJECT_FLOAT = 1    /* sigma += dt*dt*s in float (GPU variant, ..._concise/ElasticWaveEquation.cu:216-218) */
+    ORACLE_INJECT_FLOAT = 1,   /* sigma += dt*dt*s in float (GPU variant, ..._concise/ElasticWaveEquation.cu:216-218) */
+    /* The CUDA variant's unsplit interior keeps what locateSource added
+     * (..._concise/ElasticWaveEquation.cu:216-218 then :256-258), whereas the
+     * CPU path rebuilds sigma from its splits every step
+     * (ElasticWaveEquation3D.cpp:215,220,225) and so drops the injection after
+     * the velocity update.  This mode reproduces the CUDA behaviour: the
+     * running float sum A += dt*dt*s is added before each velocity update. */
+    ORACLE_INJECT_ACCUMULATE = 2
 };
 
 enum {

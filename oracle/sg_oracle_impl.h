@@ -118,6 +118,7 @@ static int FN(run)(const oracle_params *p, const REAL *b_in, const REAL *lam_in,
     int nthr = p->nthreads > 0 ? p->nthreads : 1;
 #endif
 
+    REAL acc_src = 0;
     for (int it = 0; it < p->nt; ++it) {
         /* loadSeismicSource (ElasticWaveEquation3D.cpp:121-130) */
         if (p->inject_mode == ORACLE_INJECT_DOUBLE) {
@@ -125,6 +126,11 @@ static int FN(run)(const oracle_params *p, const REAL *b_in, const REAL *lam_in,
             sxx[src] = (REAL)((double)sxx[src] + amp);
             syy[src] = (REAL)((double)syy[src] + amp);
             szz[src] = (REAL)((double)szz[src] + amp);
+        } else if (p->inject_mode == ORACLE_INJECT_ACCUMULATE) {
+            acc_src += dt * dt * wavelet[it];
+            sxx[src] += acc_src;
+            syy[src] += acc_src;
+            szz[src] += acc_src;
         } else {
             REAL amp = dt * dt * wavelet[it];
             sxx[src] += amp;

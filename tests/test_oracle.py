@@ -2,6 +2,11 @@
 
 * G0/G1/G2 are golden vectors written by the UNMODIFIED reference CPU solver
   (tests/golden/make_golden.py); the oracle must reproduce them bit for bit.
+* stationData_concise_gpu.txt is the one numeric file the reference ships
+  (3DGPUwithPML_1DArraySingleGPU_concise/stationData.txt, written by its CUDA
+  solver).  Its rows before any PML interaction pin the interior stencil, the
+  source (with the CUDA variant's accumulating injection), the time order and
+  the recording.
 * Closed forms and invariants pin what those leave open.
 """
 import math
@@ -132,6 +137,44 @@ def test_g2_config1_traces_bitexact(oracle_mod):
     assert np.array_equal(F[:, 2::5, 2::5, 2::5], g["fields_sub"])
     for f in range(9):
         assert np.sum(F[f].astype(np.float64) ** 2) == pytest.approx(meta["field_sumsq"][f], rel=1e-12)
+
+
+def test_reference_gpu_station_file(oracle_mod):
+    """Rows 0..249 of the reference CUDA run.
+
+    Reference setup (..._concise/main.cu:76-120): 300x200x100 total, pml 10,
+    dt 0.5 ms, dx 10 m, f0 20 Hz, rho 2600, lambda = rho*5000^2, mu = lambda/3,
+    source grid/2, station grid/2 + pml, sxx recorded.  The nearest PML face is
+    35 cells from the source (z), so a 100^3 box with the same source-station
+    offset (10,10,10) records the same signal until the first PML interaction
+    (after row 250).  The CUDA solver keeps the injected stress in its unsplit
+    interior, hence ORACLE_INJECT_ACCUMULATE; the file holds 6 significant
+    digits of an FMA-contracted unsplit evaluation, hence a tolerance.
+    """
+    import os
+    from conftest import GOLDEN
+
+    O = oracle_mod
+    ref = np.loadtxt(os.path.join(GOLDEN, "stationData_concise_gpu.txt"), dtype=np.float64)
+    assert ref.shape == (1500,)
+    nt, N = 250, 100
+    rho = np.float32(2600.0)
+    lam0 = np.float32(2600.0 * 5000.0 ** 2)
+    shape = (N, N, N)
+    b = np.full(shape, np.float32(1.0) / rho, np.float32)
+    lam = np.full(shape, lam0, np.float32)
+    mu = np.full(shape, np.float32(lam0 / np.float32(3.0)), np.float32)
+    p = O.OracleParams(N, N, N, 10, 5e-4, 10.0, 10.0, 10.0, nt, (50, 50, 50), coef=O.REF_COEF,
+                       inject_mode=O.INJECT_ACCUMULATE)
+    w = O.ricker(20.0, 5e-4, nt, O.RICKER_CEIL_DELAY)
+    tr, _ = O.run(p, b, lam, mu, w, [[60, 60, 60]], comp_mask=1 << 3, fields=False)
+    got = tr[:, 0, 0].astype(np.float64)
+    assert np.argmax(got) == np.argmax(ref[:nt]) == 235
+    assert _rel_l2(got, ref[:nt]) < 1e-4
+    # the CPU path's transient injection gives a different (un-integrated) signal
+    p.inject_mode = O.INJECT_FLOAT
+    tr2, _ = O.run(p, b, lam, mu, w, [[60, 60, 60]], comp_mask=1 << 3, fields=False)
+    assert _rel_l2(tr2[:, 0, 0], ref[:nt]) > 0.5
 
 
 # ------------------------------------------------------------------ invariants
