@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Throughput of the staggered-grid elastic solver on one MI355X.
+
+Workload (default): the 512x512x384 four-layer + 5 % random medium with a
+20-cell PML, dt 0.8 ms, FP32, the configuration the stated target is quoted on.
+A step is one full time step (source, velocity sweep, stress sweep, station
+gather) over the whole grid; inputs are resident in HBM before timing.
+
+Prints one JSON line: Gpt-updates/s over the full grid (PML and halo
+included), the roofline position of the dominant sweep (HIP-event timed over
+the timed region, algorithmic bytes from the lean split-PML word count), and
+the CPU oracle timed on a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "staggered-grid_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "Gpt-updates/s + % HBM roofline; FP16-store vs FP32 at fixed seismogram error"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, planes=60, steps=4):
+    """The C oracle (single thread) on the first `planes` z-planes of the same
+    model (same x/y extent and PML), `steps` time steps."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    nz = min(planes, wl.nz)
+    p = O.OracleParams(wl.nx, wl.ny, nz, wl.pml, wl.dt, wl.dx, wl.dy, wl.dz, steps,
+                       (wl.src[0], wl.src[1], min(wl.src[2], nz - 6)), coef=tuple(float(c) for c in wl.coef),
+                       damp_mode=O.DAMP_QUADRATIC, damp_v=wl.damp_v, nthreads=1)
+    sl = slice(0, nz)
+    b, lam, mu = (np.ascontiguousarray(a[sl]) for a in (wl.b, wl.lam, wl.mu))
+    w = np.asarray(wl.wavelet[:steps], np.float32)
+    t = time.perf_counter()
+    O.run(p, b, lam, mu, w, None, fields=False)
+    el = time.perf_counter() - t
+    pts = wl.nx * wl.ny * nz
+    return {"value": pts * steps / el / 1e9, "unit": "Gpt-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/sg_oracle.c (gcc -O2, 1 thread) on z-planes 0..{nz - 1} of the same model "
+                      f"({wl.nx}x{wl.ny}x{nz}, same PML), {steps} steps, {el:.1f} s",
+            "host_cpus": os.cpu_count()}
+
+
+def read_traffic(workload, precision):
+    """HBM bytes per stress launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, copy-kernel calibrated), if a summary is committed."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_stress_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="layered512")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16", "bf16"])
+    ap.add_argument("--kernel", default="tiled", choices=["tiled", "simple"])
+    ap.add_argument("--tile", type=int, nargs=2, default=None, metavar=("TY", "ZCHUNK"))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import sgfd
+    from sgfd import configs
+
+    if args.copy_calib:
+        gbps = sgfd.copy_bandwidth(1 << 30, 20)
+        print(json.dumps({"copy_gbps": gbps, "bytes": 1 << 30}))
+        return
+
+    log(f"bench: building workload {args.workload}")
+    t = time.perf_counter()
+    wl = configs.BUILDERS[args.workload]()
+    log(f"bench: workload ready in {time.perf_counter() - t:.1f} s, courant {wl.courant():.3f}")
+    s = wl.solver(precision=args.precision, kernel=args.kernel, time_length=args.warmup + args.steps)
+    if args.tile:
+        s.set_tile(*args.tile)
+    log(f"bench: warmup {args.warmup} steps")
+    s.step(args.warmup)
+    s.synchronize()
+    log(f"bench: timing {args.steps} steps")
+    ms_v, ms_s, ms_tot = s.step_timed(args.steps)
+    s.synchronize()
+    vbytes, sbytes = s.sweep_bytes()
+    energy = s.energy()
+    s.close()
+
+    K = args.steps
+    sec = ms_tot / 1e3
+    value = wl.points * K / sec / 1e9
+    avg_v, avg_s = ms_v / K, ms_s / K
+    dom = ("stress", avg_s, sbytes) if avg_s >= avg_v else ("velocity", avg_v, vbytes)
+    achieved = dom[2] / (dom[1] * 1e-3) / 1e9
+    traffic = read_traffic(args.workload, args.precision)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_tot / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": wl.description, "grid": [wl.nx, wl.ny, wl.nz], "pml": wl.pml,
+                   "precision": args.precision, "kernel": args.kernel, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "alg_bytes_per_launch": dom[2], "avg_launch_ms": dom[1]},
+        "sweeps_ms": {"velocity_avg": avg_v, "stress_avg": avg_s},
+        "step_hbm_frac": (vbytes + sbytes) * K / sec / 1e9 / HBM_PEAK_GBPS,
+        "energy_after": list(energy),
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle, bounded sample)")
+        out["cpu_baseline"] = cpu_baseline(wl)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,215 @@
+/*
+ * sgfd.h -- C ABI of the MI355X (gfx950) staggered-grid elastic solver.
+ *
+ * One solver = one 3-D velocity-stress model on one GPU: 10th-order
+ * staggered-grid derivatives, leapfrog time stepping, split-field PML, an
+ * explosive point source on sxx/syy/szz and station recording.  All device
+ * memory is owned by the solver; every call returns an sg_status and never
+ * exits the process.  A solver owns one HIP stream; different solvers are
+ * independent (no global device state), so several may coexist.
+ *
+ * Reference interfaces replaced (paths relative to the reference root,
+ * CONCISE = 3DGPUwithPML_1DArraySingleGPU_concise,
+ * CPUPML  = 3DCPUwithPML_3DArray_GridIncludePMLandOrder):
+ *   ElasticWaveEquation ctor  CONCISE/ElasticWaveEquation.h:41-46, .cu:8-45
+ *   ElasticWaveEquation3D ctor CPUPML/ElasticWaveEquation3D.h:13-18
+ *   run()                     CONCISE/ElasticWaveEquation.cu:140-185
+ *   propagate(it, s)          CONCISE/ElasticWaveEquation.cu:194-206
+ *   velocityUpdate kernel     CONCISE/ElasticWaveEquation.cu:237-517
+ *   stressUpdate kernel       CONCISE/ElasticWaveEquation.cu:519-820
+ *   locateSource kernel       CONCISE/ElasticWaveEquation.cu:209-224
+ *   recordStationData kernel  CONCISE/ElasticWaveEquation.cu:226-235
+ *   sourceFunction            CPUPML/ElasticWaveEquation3D.cpp:114-119
+ *   calculateDamping          CPUPML/ElasticWaveEquation3D.cpp:253-300
+ *
+ * Layout conventions: arrays are x-fastest over the TOTAL grid (index
+ * i + j*nx + k*nx*ny) like CONCISE/ElasticMacro.h:22, where the total extent
+ * includes the 5-cell stencil halo (never updated, kept at zero) and the PML.
+ * Updated points: [5, n-5) per axis; PML: the first/last `pml` of those.
+ */
+#ifndef SGFD_H
+#define SGFD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_API_VERSION 1
+
+typedef struct sg_solver sg_solver;
+typedef struct sg_batch sg_batch;
+
+/* Status codes (the reference exits via CHECK, common.h:6-16; this returns). */
+typedef enum sg_status {
+    SG_OK = 0,
+    SG_ERR_INVALID = -1,  /* bad argument or configuration */
+    SG_ERR_RANGE = -2,    /* index (source, station, field) outside the grid */
+    SG_ERR_NOMEM = -3,    /* device or host allocation failed */
+    SG_ERR_HIP = -4,      /* a HIP runtime call failed (see sg_last_error) */
+    SG_ERR_STATE = -5,    /* call not valid in the current state */
+    SG_ERR_UNSUPPORTED = -6
+} sg_status;
+
+typedef enum sg_precision {
+    SG_FP32 = 0,  /* float storage, float arithmetic */
+    SG_FP64 = 1,  /* double storage, double arithmetic (the error yardstick) */
+    SG_FP16 = 2,  /* IEEE half storage with per-field power-of-two scales, float arithmetic */
+    SG_BF16 = 3   /* bfloat16 storage with per-field power-of-two scales, float arithmetic */
+} sg_precision;
+
+/* Source semantics.  TRANSIENT follows the reference CPU path: the injected
+ * dt^2*s(it) is seen by that step's velocity update only, because the stress
+ * update rebuilds sigma from its splits (CPUPML/ElasticWaveEquation3D.cpp:
+ * 121-130 then 215/220/225).  ACCUMULATE follows the reference CUDA path,
+ * whose unsplit interior keeps every injection (CONCISE/.cu:216-218, 256-258),
+ * with the float add dt*dt*s of locateSource. */
+typedef enum sg_source_mode { SG_SOURCE_TRANSIENT = 0, SG_SOURCE_ACCUMULATE = 1 } sg_source_mode;
+
+/* Field ids; also the bit positions of station component masks. */
+typedef enum sg_field {
+    SG_VX = 0, SG_VY, SG_VZ, SG_SXX, SG_SYY, SG_SZZ, SG_SXY, SG_SXZ, SG_SYZ, SG_NFIELDS
+} sg_field;
+
+/* Kernel variants for the two half-step sweeps. */
+typedef enum sg_kernel { SG_KERNEL_TILED = 0, SG_KERNEL_SIMPLE = 1 } sg_kernel;
+
+/* Damping profile formulas for sg_damping_profile(). */
+typedef enum sg_damping_kind {
+    SG_DAMP_REFERENCE = 0, /* log10(1/R)*tau*V/(P*h)*(dist/P)^2  (CPUPML/ElasticMacro.h:21-23) */
+    SG_DAMP_QUADRATIC = 1  /* -3*V*ln(R)/(2*P*h)*(dist/P)^2 */
+} sg_damping_kind;
+
+/* Ricker delay conventions for sg_ricker(). */
+typedef enum sg_ricker_delay {
+    SG_RICKER_CEIL_DELAY = 0, /* t0 = ceil(1/(f0*dt)) steps (CPUPML/ElasticWaveEquation3D.cpp:116) */
+    SG_RICKER_NO_DELAY = 1    /* peak at t = 1/f0 */
+} sg_ricker_delay;
+
+/* Solver configuration: GRID + pml + Delta + DifferenceCoefficient + timeLength
+ * of CONCISE/ElasticWaveEquation.h:41-46 (GPUDim is replaced by `kernel` and
+ * the tile choice; the source and stations are set by their own calls). */
+typedef struct sg_config {
+    int nx, ny, nz;        /* total extents incl. PML and the 5-cell halo */
+    int pml;               /* PML thickness in cells (0 = no absorbing layer) */
+    float dt, dx, dy, dz;  /* time step [s] and spacings [m] */
+    float diff_coef[5];    /* signed staggered coefficients c1..c5 */
+    int time_length;       /* steps run by sg_run(); also the station buffer length */
+    int precision;         /* sg_precision */
+    int source_mode;       /* sg_source_mode */
+    int kernel;            /* sg_kernel */
+} sg_config;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Allocates device state (fields, PML splits, tables) on the current HIP
+ * device.  Fields start at zero; damping defaults to the reference CPU
+ * formula (tau 3.4, V 2000, R 1e-3) until sg_set_damping is called. */
+int sg_create(const sg_config *cfg, sg_solver **out);
+int sg_destroy(sg_solver *s);
+
+/* Host-side copy of the model (Medium buoyancy/lambda/mu of CONCISE/
+ * ElasticMacro.h:379-384, copied in like run()'s H2D at .cu:145-147).
+ * x-fastest over the total grid, n = nx*ny*nz values each. */
+int sg_set_medium(sg_solver *s, const float *buoyancy, const float *lambda, const float *mu);
+
+/* Per-axis damping d(i) [1/s] for i in [0, n) of that axis (values outside
+ * the PML slabs are ignored).  The update factors (1 -/+ dt*d/2) are formed
+ * in double on the host. */
+int sg_set_damping(sg_solver *s, const double *d_x, const double *d_y, const double *d_z);
+
+/* Point source at absolute index (x, y, z) (SOURCELOCATION) with wavelet
+ * samples s(it), it < n (steps beyond n inject nothing).  The injected
+ * amount is pow(dt,2)*s(it) in double (TRANSIENT) or dt*dt*s(it) in the
+ * storage precision (ACCUMULATE). */
+int sg_set_source(sg_solver *s, int x, int y, int z, const float *wavelet, int n);
+
+/* Overwrites wavelet sample `it` (it < n of sg_set_source): the source value
+ * argument of propagate(timeIndex, sourceValue) (CONCISE/.cu:194). */
+int sg_set_source_sample(sg_solver *s, int it, float value);
+
+/* Stations (STATIONLOCATION, generalised to many) and the recorded
+ * components (bit f of comp_mask records field f).  Recorded after the
+ * stress update of every `interval`-th step (interval >= 1), step-major:
+ * [record][station][component].  Replaces any previous station set. */
+int sg_set_stations(sg_solver *s, const int *xyz, int n, unsigned comp_mask, int interval);
+
+/* ---- stepping --------------------------------------------------------------- */
+
+/* Advances nsteps (source -> velocity -> stress -> record per step), queued on
+ * the solver's stream; returns without waiting. */
+int sg_step(sg_solver *s, int nsteps);
+/* time_length steps from the current state, then waits (run(), minus its
+ * file output). */
+int sg_run(sg_solver *s);
+int sg_synchronize(sg_solver *s);
+/* Zeroes fields, splits and the step counter (the model is kept). */
+int sg_reset(sg_solver *s);
+/* Steps completed (queued) so far. */
+long long sg_steps_done(const sg_solver *s);
+
+/* ---- results ------------------------------------------------------------- */
+
+/* Copies up to `count` recorded station values (float; waits for the stream).
+ * Returns the number of values available in *n_out. */
+int sg_get_stations(sg_solver *s, float *out, long long count, long long *n_out);
+/* Copies one full field (x-fastest, total grid) to host float / double. */
+int sg_get_field(sg_solver *s, int field, float *out);
+int sg_get_field_f64(sg_solver *s, int field, double *out);
+/* FP64 reductions over the updated region: out[0] = sum rho*|v|^2/2,
+ * out[1] = sum of the six stress components squared.  Used to check that
+ * the PML drains energy after the source stops. */
+int sg_energy(sg_solver *s, double out[2]);
+
+/* ---- timing and tuning -------------------------------------------------------- */
+
+/* Like sg_step but brackets each sweep with HIP events on the solver stream;
+ * returns the summed milliseconds of the velocity sweeps, the stress sweeps
+ * and the whole interval (waits for the stream). */
+int sg_step_timed(sg_solver *s, int nsteps, double *ms_velocity, double *ms_stress, double *ms_total);
+/* Captures one time step into a hipGraph and replays it (1 = on, 0 = off). */
+int sg_set_graph(sg_solver *s, int enable);
+/* Tile shape of the tiled sweeps: y rows per block (4, 8 or 16) and z planes
+ * per block (>= 1; 0 = whole updated depth). */
+int sg_set_tile(sg_solver *s, int tile_y, int z_chunk);
+/* Algorithmic HBM bytes of one velocity and one stress sweep (the lean
+ * split-PML word count times the storage element size). */
+int sg_sweep_bytes(const sg_solver *s, double *velocity_bytes, double *stress_bytes);
+/* Device copy of `bytes` between two solver-owned scratch buffers, timed with
+ * HIP events: the measured-copy yardstick for bandwidth fractions. */
+int sg_copy_bandwidth(long long bytes, int reps, double *gbps);
+
+/* ---- batched shots ------------------------------------------------------- */
+
+/* Groups existing FP32 solvers (any sizes) so one launch per half-step
+ * advances all of them over a cross-shot tile work-list.  The solvers stay
+ * owned by the caller and must outlive the batch. */
+int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out);
+int sg_batch_step(sg_batch *b, int nsteps);
+int sg_batch_synchronize(sg_batch *b);
+int sg_batch_destroy(sg_batch *b);
+
+/* ---- host numerics (FP64, no device work) --------------------------------- */
+
+/* s(it) of sourceFunction in the reference's float arithmetic. */
+int sg_ricker(float f0, float dt, int nt, int delay_mode, float *out);
+/* d(i), i < n, for one axis of total extent n (sg_damping_kind). */
+int sg_damping_profile(int n, int pml, double h, int kind, double v, double r, double tau, double *out);
+/* Coefficients: kind 0 = the reference literals (CPUPML/ElasticMacro.h:65-69),
+ * kind 1 = the exact rationals 19845/16384, -735/8192, 567/40960,
+ * -405/229376, 35/294912. */
+int sg_diff_coef(int kind, float out[5]);
+
+/* Unit-variance Gaussian-correlated random field (correlation exp(-r^2/a^2),
+ * a = corr cells), seeded and reproducible, clipped to +-clip; x-fastest. */
+int sg_gaussian_field(int nx, int ny, int nz, double corr, unsigned long long seed, double clip, float *out);
+
+/* Last error message of the calling thread ("" if none). */
+const char *sg_last_error(void);
+int sg_api_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGFD_H */

@@ -1,0 +1,119 @@
+// sg_internal.hpp -- device-side views and launch entry points shared by the
+// HIP kernels (sg_kernels.hip, sg_tiled.hip) and the solver (sg_solver.cpp).
+//
+// Data layout in HBM (one shot):
+//   * 9 wavefields and 3 model arrays, x-fastest over the total grid
+//     (index i + j*sy + k*sz, sy = nx, sz = nx*ny), like the reference CUDA
+//     INDEX macro (3DGPUwithPML_1DArraySingleGPU_concise/ElasticMacro.h:22).
+//   * PML split components only where a split exists: a compact slab store
+//     (z-slabs, then y-slabs of the z-interior, then x-slabs of the y/z
+//     interior), x-contiguous inside each slab.  Every split array spans the
+//     whole store; a split is touched only at points where its axis damps.
+//   * per-axis damping factor tables A(i) = (1 - dt d/2)/(1 + dt d/2) and
+//     B(i) = 1/(1 + dt d/2), formed in double on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sg {
+
+constexpr int HALO = 5;  // half stencil width: updated points are [5, n-5)
+
+enum Field { VX = 0, VY, VZ, SXX, SYY, SZZ, SXY, SXZ, SYZ, NFIELD };
+
+// Stress split slots: normal components carry x,y,z splits, shear components
+// only the two axes that contribute (the reference's third shear split is
+// identically zero: 3DCPUwithPML.../ElasticWaveEquation3D.cpp:233,237,241).
+enum StressSplit {
+    XX_X = 0, XX_Y, XX_Z, YY_X, YY_Y, YY_Z, ZZ_X, ZZ_Y, ZZ_Z,
+    XY_X, XY_Y, XZ_X, XZ_Z, YZ_Y, YZ_Z, NSSPLIT
+};
+
+struct Geom {
+    int nx, ny, nz;      // total extents
+    int pml;             // PML thickness
+    int ux, uy, uz;      // updated extents (n - 10)
+    long long sy, sz;    // strides in elements
+    long long zsec;      // end of the z-slab section of the split store
+    long long ysec;      // end of the y-slab section
+    long long nslots;    // total split slots
+};
+
+__host__ __device__ inline bool in_pml(int i, int n, int pml)
+{
+    return i < HALO + pml || i >= n - HALO - pml;
+}
+
+// Compact split-store slot of an updated PML point (i, j, k) (absolute).
+__host__ __device__ inline long long split_slot(const Geom &g, int i, int j, int k)
+{
+    const int P = g.pml;
+    const int ii = i - HALO, jj = j - HALO, kk = k - HALO;
+    if (kk < P || kk >= g.uz - P) {
+        const int kz = kk < P ? kk : kk - (g.uz - 2 * P);
+        return ((long long)kz * g.uy + jj) * g.ux + ii;
+    }
+    if (jj < P || jj >= g.uy - P) {
+        const int jy = jj < P ? jj : jj - (g.uy - 2 * P);
+        return g.zsec + ((long long)(kk - P) * (2 * P) + jy) * g.ux + ii;
+    }
+    const int ix = ii < P ? ii : ii - (g.ux - 2 * P);
+    return g.ysec + ((long long)(kk - P) * (g.uy - 2 * P) + (jj - P)) * (2 * P) + ix;
+}
+
+// Everything one sweep needs about one shot, passed by value to kernels (or
+// read from device memory by the batched kernels).
+template <class T, class C>
+struct Shot {
+    Geom g;
+    T *f[NFIELD];
+    const T *b, *lam, *mu;
+    C *sv[9];           // velocity splits: component*3 + axis
+    C *ss[NSSPLIT];     // stress splits
+    const C *dA[3];     // damping factor tables per axis
+    const C *dB[3];
+    C cx[5], cy[5], cz[5];  // c_n / h per axis
+    C dt;
+    long long src;      // linear index of the source point, -1 if none
+    const T *src_saved; // TRANSIENT mode: sxx/syy/szz at src before injection
+    // storage scales (FP16/BF16: stored = value * scale); 1 otherwise
+    float fscale[NFIELD];
+    float finv[NFIELD];
+    float bscale, lscale, mscale;  // model storage scales (value = stored / scale)
+};
+
+// Tile description of the tiled sweeps.
+struct Tile {
+    int ty;       // y rows per block (points per thread column = ty / 4)
+    int zchunk;   // z planes per block
+};
+
+// Work item of the batched launches: one (shot, x tile, y tile, z chunk).
+struct WorkItem {
+    int shot;
+    int i0, j0, k0, k1;
+};
+
+}  // namespace sg
+
+// ---- launch entry points (defined in the .hip files) -----------------------
+// prec: 0 fp32, 1 fp64, 2 fp16, 3 bf16.  `shot` points to a host Shot<T,C>
+// of the matching instantiation.
+namespace sg {
+hipError_t launch_velocity(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st);
+hipError_t launch_stress(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st);
+hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc,
+                         int namp, int accumulate, int *d_step, void *saved, hipStream_t st);
+hipError_t launch_gather(int prec, const void *shot, const long long *idx, const int *comp, int nst,
+                         int ncomp, int interval, long long capacity, void *out, const int *d_step,
+                         hipStream_t st);
+hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st);
+hipError_t launch_copy(const float *src, float *dst, long long n, hipStream_t st);
+hipError_t launch_fill_zero(void *p, long long bytes, hipStream_t st);
+// batched FP32 tiled sweeps over a work-list (shots/items in device memory)
+hipError_t launch_velocity_batch(const void *shots_dev, const WorkItem *items, int nitems, int grid,
+                                 int ty, hipStream_t st);
+hipError_t launch_stress_batch(const void *shots_dev, const WorkItem *items, int nitems, int grid,
+                               int ty, hipStream_t st);
+}  // namespace sg

@@ -1,0 +1,435 @@
+// sg_kernels.hip -- per-point ("simple") sweeps and the small kernels of a
+// time step: source injection, station gather, FP64 energy, copy yardstick.
+//
+// The simple sweeps compute one grid point per thread straight from global
+// memory.  They are the plain statement of the update that the tiled sweeps
+// (sg_tiled.hip) must reproduce, and a selectable kernel variant.
+//
+// Update being computed (3DCPUwithPML_3DArray_GridIncludePMLandOrder/
+// ElasticWaveEquation3D.cpp:134-249, with the interior written unsplit as in
+// 3DGPUwithPML_1DArraySingleGPU_concise/ElasticWaveEquation.cu:244-282,
+// 526-585):
+//   vx += dt*bx*(D+x sxx + D-y sxy + D-z sxz), bx = (b(i)+b(i+1))/2, ...
+//   sxx += dt*((lam+2mu) D-x vx + lam (D-y vy + D-z vz)), ...
+//   sxy += dt*muxy*(D+x vy + D+y vx), muxy = 4/(1/mu + ...) over 4 nodes, ...
+// At PML points every damped axis a of a component keeps a split
+//   f_a <- A(a) f_a + B(a) inc_a,   A = (1-dt d/2)/(1+dt d/2), B = 1/(1+dt d/2)
+// and the component is rebuilt as (f - sum old damped splits) + undamped
+// increments + sum new damped splits, or as the plain sum of its splits when
+// every contributing axis is damped.
+#include "sg_internal.hpp"
+#include "sg_precision.hpp"
+#include "sg_stencil.hpp"
+
+namespace sg {
+
+// ---------------------------------------------------------------------------
+// simple sweeps
+// ---------------------------------------------------------------------------
+
+template <class T, class C>
+__device__ __forceinline__ C ldf(const Shot<T, C> &s, int f, long long o)
+{
+    return Cvt<T, C>::ld(s.f[f][o], s.finv[f]);
+}
+
+template <class T, class C>
+__device__ __forceinline__ C dfwd_g(const Shot<T, C> &s, int f, long long o, long long st, const C *c)
+{
+    const T *p = s.f[f];
+    const float inv = s.finv[f];
+    auto L = [&](long long q) { return Cvt<T, C>::ld(p[q], inv); };
+    C r = c[0] * (L(o + st) - L(o));
+    r += c[1] * (L(o + 2 * st) - L(o - st));
+    r += c[2] * (L(o + 3 * st) - L(o - 2 * st));
+    r += c[3] * (L(o + 4 * st) - L(o - 3 * st));
+    r += c[4] * (L(o + 5 * st) - L(o - 4 * st));
+    return r;
+}
+
+template <class T, class C>
+__device__ __forceinline__ C dbwd_g(const Shot<T, C> &s, int f, long long o, long long st, const C *c)
+{
+    const T *p = s.f[f];
+    const float inv = s.finv[f];
+    auto L = [&](long long q) { return Cvt<T, C>::ld(p[q], inv); };
+    C r = c[0] * (L(o) - L(o - st));
+    r += c[1] * (L(o + st) - L(o - 2 * st));
+    r += c[2] * (L(o + 2 * st) - L(o - 3 * st));
+    r += c[3] * (L(o + 3 * st) - L(o - 4 * st));
+    r += c[4] * (L(o + 4 * st) - L(o - 5 * st));
+    return r;
+}
+
+template <class T, class C>
+__global__ __launch_bounds__(256) void k_velocity_simple(const Shot<T, C> s)
+{
+    const Geom &g = s.g;
+    const int i = blockIdx.x * 64 + threadIdx.x + HALO;
+    const int j = blockIdx.y * 4 + threadIdx.y + HALO;
+    const int k = blockIdx.z + HALO;
+    if (i >= g.nx - HALO || j >= g.ny - HALO) return;
+    const long long sy = g.sy, sz = g.sz;
+    const long long o = i + j * sy + k * sz;
+
+    C d[3][3];
+    d[0][0] = dfwd_g(s, SXX, o, 1, s.cx);
+    d[0][1] = dbwd_g(s, SXY, o, sy, s.cy);
+    d[0][2] = dbwd_g(s, SXZ, o, sz, s.cz);
+    d[1][0] = dbwd_g(s, SXY, o, 1, s.cx);
+    d[1][1] = dfwd_g(s, SYY, o, sy, s.cy);
+    d[1][2] = dbwd_g(s, SYZ, o, sz, s.cz);
+    d[2][0] = dbwd_g(s, SXZ, o, 1, s.cx);
+    d[2][1] = dbwd_g(s, SYZ, o, sy, s.cy);
+    d[2][2] = dfwd_g(s, SZZ, o, sz, s.cz);
+
+    const float ib = 1.0f / s.bscale;
+    const C b0 = Cvt<T, C>::ld(s.b[o], ib);
+    C db[3];
+    db[0] = s.dt * ((b0 + Cvt<T, C>::ld(s.b[o + 1], ib)) * (C)0.5);
+    db[1] = s.dt * ((b0 + Cvt<T, C>::ld(s.b[o + sy], ib)) * (C)0.5);
+    db[2] = s.dt * ((b0 + Cvt<T, C>::ld(s.b[o + sz], ib)) * (C)0.5);
+
+    const bool px = in_pml(i, g.nx, g.pml), py = in_pml(j, g.ny, g.pml), pz = in_pml(k, g.nz, g.pml);
+    if (!(px || py || pz)) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            C v = ldf(s, VX + c, o);
+            v += db[c] * (d[c][0] + d[c][1] + d[c][2]);
+            s.f[VX + c][o] = Cvt<T, C>::st(v, s.fscale[VX + c]);
+        }
+        return;
+    }
+    const long long slot = split_slot(g, i, j, k);
+    const bool dm[3] = {px, py, pz};
+    const C A[3] = {s.dA[0][i], s.dA[1][j], s.dA[2][k]};
+    const C B[3] = {s.dB[0][i], s.dB[1][j], s.dB[2][k]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const C inc[3] = {db[c] * d[c][0], db[c] * d[c][1], db[c] * d[c][2]};
+        C *const sp[3] = {s.sv[3 * c], s.sv[3 * c + 1], s.sv[3 * c + 2]};
+        const C v = pml_update<C, 3>(ldf(s, VX + c, o), inc, dm, sp, slot, A, B);
+        s.f[VX + c][o] = Cvt<T, C>::st(v, s.fscale[VX + c]);
+    }
+}
+
+template <class T, class C>
+__global__ __launch_bounds__(256) void k_stress_simple(const Shot<T, C> s)
+{
+    const Geom &g = s.g;
+    const int i = blockIdx.x * 64 + threadIdx.x + HALO;
+    const int j = blockIdx.y * 4 + threadIdx.y + HALO;
+    const int k = blockIdx.z + HALO;
+    if (i >= g.nx - HALO || j >= g.ny - HALO) return;
+    const long long sy = g.sy, sz = g.sz;
+    const long long o = i + j * sy + k * sz;
+
+    const C dX = dbwd_g(s, VX, o, 1, s.cx);
+    const C dY = dbwd_g(s, VY, o, sy, s.cy);
+    const C dZ = dbwd_g(s, VZ, o, sz, s.cz);
+    const C dxy_x = dfwd_g(s, VY, o, 1, s.cx), dxy_y = dfwd_g(s, VX, o, sy, s.cy);
+    const C dxz_x = dfwd_g(s, VZ, o, 1, s.cx), dxz_z = dfwd_g(s, VX, o, sz, s.cz);
+    const C dyz_y = dfwd_g(s, VZ, o, sy, s.cy), dyz_z = dfwd_g(s, VY, o, sz, s.cz);
+
+    const float il = 1.0f / s.lscale, im = 1.0f / s.mscale;
+    const C lam = Cvt<T, C>::ld(s.lam[o], il);
+    auto M = [&](long long q) { return Cvt<T, C>::ld(s.mu[q], im); };
+    const C mu = M(o);
+    const C r0 = (C)1 / mu, rx = (C)1 / M(o + 1), ry = (C)1 / M(o + sy), rz = (C)1 / M(o + sz);
+    const C rxy = (C)1 / M(o + 1 + sy), rxz = (C)1 / M(o + 1 + sz), ryz = (C)1 / M(o + sy + sz);
+    const C muxy = (C)4 / (((r0 + rx) + ry) + rxy);
+    const C muxz = (C)4 / (((r0 + rx) + rz) + rxz);
+    const C muyz = (C)4 / (((r0 + ry) + rz) + ryz);
+
+    const C dl = s.dt * lam, dl2m = s.dt * (lam + (C)2 * mu);
+    const C dmxy = s.dt * muxy, dmxz = s.dt * muxz, dmyz = s.dt * muyz;
+
+    C old[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) old[c] = ldf(s, SXX + c, o);
+    if (o == s.src && s.src_saved) {
+        // TRANSIENT source: the stress update starts from sigma before injection
+#pragma unroll
+        for (int c = 0; c < 3; ++c) old[c] = Cvt<T, C>::ld(s.src_saved[c], s.finv[SXX + c]);
+    }
+
+    const bool px = in_pml(i, g.nx, g.pml), py = in_pml(j, g.ny, g.pml), pz = in_pml(k, g.nz, g.pml);
+    C nw[6];
+    if (!(px || py || pz)) {
+        nw[0] = old[0] + (dl2m * dX + dl * (dY + dZ));
+        nw[1] = old[1] + (dl2m * dY + dl * (dX + dZ));
+        nw[2] = old[2] + (dl2m * dZ + dl * (dX + dY));
+        nw[3] = old[3] + dmxy * (dxy_x + dxy_y);
+        nw[4] = old[4] + dmxz * (dxz_x + dxz_z);
+        nw[5] = old[5] + dmyz * (dyz_y + dyz_z);
+    } else {
+        const long long slot = split_slot(g, i, j, k);
+        const bool dm[3] = {px, py, pz};
+        const C A[3] = {s.dA[0][i], s.dA[1][j], s.dA[2][k]};
+        const C B[3] = {s.dB[0][i], s.dB[1][j], s.dB[2][k]};
+        {
+            const C inc[3] = {dl2m * dX, dl * dY, dl * dZ};
+            C *const sp[3] = {s.ss[XX_X], s.ss[XX_Y], s.ss[XX_Z]};
+            nw[0] = pml_update<C, 3>(old[0], inc, dm, sp, slot, A, B);
+        }
+        {
+            const C inc[3] = {dl * dX, dl2m * dY, dl * dZ};
+            C *const sp[3] = {s.ss[YY_X], s.ss[YY_Y], s.ss[YY_Z]};
+            nw[1] = pml_update<C, 3>(old[1], inc, dm, sp, slot, A, B);
+        }
+        {
+            const C inc[3] = {dl * dX, dl * dY, dl2m * dZ};
+            C *const sp[3] = {s.ss[ZZ_X], s.ss[ZZ_Y], s.ss[ZZ_Z]};
+            nw[2] = pml_update<C, 3>(old[2], inc, dm, sp, slot, A, B);
+        }
+        {
+            const C inc[2] = {dmxy * dxy_x, dmxy * dxy_y};
+            const bool m2[2] = {px, py};
+            C *const sp[2] = {s.ss[XY_X], s.ss[XY_Y]};
+            const C A2[2] = {A[0], A[1]}, B2[2] = {B[0], B[1]};
+            nw[3] = pml_update<C, 2>(old[3], inc, m2, sp, slot, A2, B2);
+        }
+        {
+            const C inc[2] = {dmxz * dxz_x, dmxz * dxz_z};
+            const bool m2[2] = {px, pz};
+            C *const sp[2] = {s.ss[XZ_X], s.ss[XZ_Z]};
+            const C A2[2] = {A[0], A[2]}, B2[2] = {B[0], B[2]};
+            nw[4] = pml_update<C, 2>(old[4], inc, m2, sp, slot, A2, B2);
+        }
+        {
+            const C inc[2] = {dmyz * dyz_y, dmyz * dyz_z};
+            const bool m2[2] = {py, pz};
+            C *const sp[2] = {s.ss[YZ_Y], s.ss[YZ_Z]};
+            const C A2[2] = {A[1], A[2]}, B2[2] = {B[1], B[2]};
+            nw[5] = pml_update<C, 2>(old[5], inc, m2, sp, slot, A2, B2);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s.f[SXX + c][o] = Cvt<T, C>::st(nw[c], s.fscale[SXX + c]);
+}
+
+// ---------------------------------------------------------------------------
+// source injection (one thread): also advances the device step counter
+// ---------------------------------------------------------------------------
+// TRANSIENT: sigma = float(double(sigma) + pow(dt,2)*s(it)) for sxx/syy/szz
+// after saving the pre-injection values (3DCPUwithPML.../ElasticWaveEquation3D.cpp:
+// 123-125).  ACCUMULATE: sigma += dt*dt*s(it) in the storage precision
+// (3DGPUwithPML_1DArraySingleGPU_concise/ElasticWaveEquation.cu:216-218).
+template <class T, class C>
+__global__ void k_inject(const Shot<T, C> s, const double *amp_d, const C *amp_acc, int namp, int accumulate,
+                         int *d_step, T *saved)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int it = *d_step;
+    *d_step = it + 1;
+    if (s.src < 0) return;
+    for (int c = 0; c < 3; ++c) {
+        T *p = s.f[SXX + c] + s.src;
+        const C cur = Cvt<T, C>::ld(*p, s.finv[SXX + c]);
+        if (accumulate) {
+            if (it < namp) *p = Cvt<T, C>::st(cur + amp_acc[it], s.fscale[SXX + c]);
+        } else {
+            saved[c] = *p;
+            if (it < namp) *p = Cvt<T, C>::st((C)((double)cur + amp_d[it]), s.fscale[SXX + c]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// station gather: one thread per (station, component) after the stress sweep
+// ---------------------------------------------------------------------------
+template <class T, class C>
+__global__ void k_gather(const Shot<T, C> s, const long long *idx, const int *comp, int nst, int ncomp,
+                         int interval, long long cap, C *out, const int *d_step)
+{
+    const int it = *d_step - 1;
+    if (it < 0 || it % interval) return;
+    const long long row = it / interval;
+    if (row >= cap) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nst * ncomp) return;
+    const int st = t / ncomp, q = t - st * ncomp;
+    const int f = comp[q];
+    out[row * (long long)nst * ncomp + t] = Cvt<T, C>::ld(s.f[f][idx[st]], s.finv[f]);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 energy over the updated region
+// ---------------------------------------------------------------------------
+template <class T, class C>
+__global__ __launch_bounds__(256) void k_energy(const Shot<T, C> s, double *out)
+{
+    const Geom &g = s.g;
+    const long long n = (long long)g.ux * g.uy * g.uz;
+    double ek = 0.0, es = 0.0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % g.ux) + HALO;
+        const long long r = t / g.ux;
+        const int j = (int)(r % g.uy) + HALO;
+        const int k = (int)(r / g.uy) + HALO;
+        const long long o = i + j * g.sy + k * g.sz;
+        const double b = (double)Cvt<T, C>::ld(s.b[o], 1.0f / s.bscale);
+        double v2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double v = (double)ldf(s, VX + c, o);
+            v2 += v * v;
+        }
+        ek += 0.5 * v2 / b;
+        for (int c = 0; c < 6; ++c) {
+            const double v = (double)ldf(s, SXX + c, o);
+            es += v * v;
+        }
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = ek;
+    red[1][threadIdx.x] = es;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], red[0][0]);
+        atomicAdd(&out[1], red[1][0]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n4)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n4; t += (long long)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+__global__ __launch_bounds__(256) void k_zero(uint32_t *p, long long n)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        p[t] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+
+template <class T, class C>
+static hipError_t vel_simple(const void *shot, hipStream_t st)
+{
+    const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
+    dim3 grid((s.g.ux + 63) / 64, (s.g.uy + 3) / 4, s.g.uz), block(64, 4);
+    hipLaunchKernelGGL((k_velocity_simple<T, C>), grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+template <class T, class C>
+static hipError_t str_simple(const void *shot, hipStream_t st)
+{
+    const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
+    dim3 grid((s.g.ux + 63) / 64, (s.g.uy + 3) / 4, s.g.uz), block(64, 4);
+    hipLaunchKernelGGL((k_stress_simple<T, C>), grid, block, 0, st, s);
+    return hipGetLastError();
+}
+
+}  // namespace sg
+
+#include "sg_tiled.hpp"
+
+namespace sg {
+
+#define SG_DISPATCH(prec, FN, ...)                               \
+    switch (prec) {                                              \
+    case 0: return FN<float, float>(__VA_ARGS__);                \
+    case 1: return FN<double, double>(__VA_ARGS__);              \
+    case 2: return FN<_Float16, float>(__VA_ARGS__);             \
+    case 3: return FN<bf16, float>(__VA_ARGS__);                 \
+    default: return hipErrorInvalidValue;                        \
+    }
+
+template <class T, class C>
+static hipError_t vel_any(int kernel, const void *shot, const Tile &t, hipStream_t st)
+{
+    if (kernel == 1) return vel_simple<T, C>(shot, st);
+    return velocity_tiled<T, C>(*static_cast<const Shot<T, C> *>(shot), t, st);
+}
+
+template <class T, class C>
+static hipError_t str_any(int kernel, const void *shot, const Tile &t, hipStream_t st)
+{
+    if (kernel == 1) return str_simple<T, C>(shot, st);
+    return stress_tiled<T, C>(*static_cast<const Shot<T, C> *>(shot), t, st);
+}
+
+hipError_t launch_velocity(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st)
+{
+    SG_DISPATCH(prec, vel_any, kernel, shot, t, st)
+}
+
+hipError_t launch_stress(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st)
+{
+    SG_DISPATCH(prec, str_any, kernel, shot, t, st)
+}
+
+template <class T, class C>
+static hipError_t inject_t(const void *shot, const double *amp_d, const void *amp_acc, int namp, int acc,
+                           int *d_step, void *saved, hipStream_t st)
+{
+    const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
+    hipLaunchKernelGGL((k_inject<T, C>), dim3(1), dim3(64), 0, st, s, amp_d, static_cast<const C *>(amp_acc), namp,
+                       acc, d_step, static_cast<T *>(saved));
+    return hipGetLastError();
+}
+
+hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc, int namp,
+                         int accumulate, int *d_step, void *saved, hipStream_t st)
+{
+    SG_DISPATCH(prec, inject_t, shot, amp_d, amp_acc, namp, accumulate, d_step, saved, st)
+}
+
+template <class T, class C>
+static hipError_t gather_t(const void *shot, const long long *idx, const int *comp, int nst, int ncomp,
+                           int interval, long long cap, void *out, const int *d_step, hipStream_t st)
+{
+    const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
+    const int n = nst * ncomp;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_gather<T, C>), dim3((n + 255) / 256), dim3(256), 0, st, s, idx, comp, nst, ncomp,
+                       interval, cap, static_cast<C *>(out), d_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(int prec, const void *shot, const long long *idx, const int *comp, int nst, int ncomp,
+                         int interval, long long capacity, void *out, const int *d_step, hipStream_t st)
+{
+    SG_DISPATCH(prec, gather_t, shot, idx, comp, nst, ncomp, interval, capacity, out, d_step, st)
+}
+
+template <class T, class C>
+static hipError_t energy_t(const void *shot, double *out2, hipStream_t st)
+{
+    const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
+    hipLaunchKernelGGL((k_energy<T, C>), dim3(1024), dim3(256), 0, st, s, out2);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st)
+{
+    SG_DISPATCH(prec, energy_t, shot, out2, st)
+}
+
+hipError_t launch_copy(const float *src, float *dst, long long n, hipStream_t st)
+{
+    const long long n4 = n / 4;
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, st, reinterpret_cast<const float4 *>(src),
+                       reinterpret_cast<float4 *>(dst), n4);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(void *p, long long bytes, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero, dim3(4096), dim3(256), 0, st, static_cast<uint32_t *>(p), bytes / 4);
+    return hipGetLastError();
+}
+
+}  // namespace sg

@@ -1,0 +1,758 @@
+// sg_solver.cpp -- the solver object behind the C ABI (include/sgfd.h).
+//
+// Owns every device buffer of one shot, one HIP stream, an optional hipGraph
+// of one time step and the HIP events of timed runs.  The time step is
+//   inject(it) -> velocity sweep -> stress sweep -> station gather
+// as in propagate() of the reference (3DCPUwithPML_3DArray_GridIncludePMLandOrder/
+// ElasticWaveEquation3D.cpp:59-69; 3DGPUwithPML_1DArraySingleGPU_concise/
+// ElasticWaveEquation.cu:194-206), all queued asynchronously on the solver's
+// stream; the step index lives in device memory so that one captured step
+// can be replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sgfd.h"
+#include "sg_error.hpp"
+#include "sg_internal.hpp"
+#include "sg_precision.hpp"
+
+namespace sg {
+std::string &last_error()
+{
+    static thread_local std::string e;
+    return e;
+}
+}  // namespace sg
+
+using namespace sg;
+
+namespace {
+
+size_t storage_size(int prec) { return prec == SG_FP64 ? 8 : (prec == SG_FP32 ? 4 : 2); }
+size_t compute_size(int prec) { return prec == SG_FP64 ? 8 : 4; }
+
+// bytes of one field component per point: RMW of the full value and of each
+// damped split, or -- when every contributing axis damps -- RMW of the splits
+// plus a write of the full value (the reference rebuilds it from its splits).
+double comp_bytes(int naxes, int ndamped, double sT, double sC)
+{
+    if (ndamped == naxes) return 2.0 * ndamped * sC + sT;
+    return 2.0 * sT + 2.0 * ndamped * sC;
+}
+
+}  // namespace
+
+struct sg_solver {
+    sg_config cfg{};
+    int prec = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Geom g{};
+    Tile tile{8, 0};
+
+    void *fields[NFIELD] = {};
+    void *model[3] = {};
+    void *splits = nullptr;  // 24 split arrays of g.nslots compute-type values
+    void *tables = nullptr;  // dA[3], dB[3] back to back, compute type
+    void *saved = nullptr;   // 3 storage values (TRANSIENT source)
+    int *d_step = nullptr;
+    double *amp_d = nullptr;
+    void *amp_acc = nullptr;
+    int namp = 0;
+    long long src = -1;
+
+    long long *st_idx = nullptr;
+    int *st_comp = nullptr;
+    void *st_out = nullptr;
+    int nst = 0, ncomp = 0, interval = 1;
+    long long st_cap = 0;
+
+    double *d_energy = nullptr;
+    long long steps_done = 0;
+    std::vector<double> damp[3];
+
+    bool use_graph = false;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    std::vector<hipEvent_t> events;
+
+    alignas(16) unsigned char shot[sizeof(Shot<double, double>)];
+    float fscale[NFIELD];
+    float bscale = 1, lscale = 1, mscale = 1;
+};
+
+namespace {
+
+template <class T, class C>
+void fill_shot(sg_solver *s)
+{
+    Shot<T, C> &h = *reinterpret_cast<Shot<T, C> *>(s->shot);
+    std::memset(&h, 0, sizeof h);
+    h.g = s->g;
+    for (int f = 0; f < NFIELD; ++f) h.f[f] = static_cast<T *>(s->fields[f]);
+    h.b = static_cast<const T *>(s->model[0]);
+    h.lam = static_cast<const T *>(s->model[1]);
+    h.mu = static_cast<const T *>(s->model[2]);
+    C *sp = static_cast<C *>(s->splits);
+    for (int a = 0; a < 9; ++a) h.sv[a] = sp ? sp + (size_t)a * s->g.nslots : nullptr;
+    for (int a = 0; a < NSSPLIT; ++a) h.ss[a] = sp ? sp + (size_t)(9 + a) * s->g.nslots : nullptr;
+    C *tb = static_cast<C *>(s->tables);
+    const int n3[3] = {s->g.nx, s->g.ny, s->g.nz};
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        h.dA[a] = tb + off;
+        h.dB[a] = tb + off + n3[a];
+        off += 2 * (size_t)n3[a];
+    }
+    const double hs[3] = {s->cfg.dx, s->cfg.dy, s->cfg.dz};
+    for (int n = 0; n < 5; ++n) {
+        h.cx[n] = (C)((double)s->cfg.diff_coef[n] / hs[0]);
+        h.cy[n] = (C)((double)s->cfg.diff_coef[n] / hs[1]);
+        h.cz[n] = (C)((double)s->cfg.diff_coef[n] / hs[2]);
+    }
+    h.dt = (C)s->cfg.dt;
+    h.src = s->src;
+    h.src_saved = (s->cfg.source_mode == SG_SOURCE_TRANSIENT && s->src >= 0) ? static_cast<const T *>(s->saved)
+                                                                               : nullptr;
+    for (int f = 0; f < NFIELD; ++f) {
+        h.fscale[f] = s->fscale[f];
+        h.finv[f] = 1.0f / s->fscale[f];
+    }
+    h.bscale = s->bscale;
+    h.lscale = s->lscale;
+    h.mscale = s->mscale;
+}
+
+void refresh_shot(sg_solver *s)
+{
+    switch (s->prec) {
+    case SG_FP32: fill_shot<float, float>(s); break;
+    case SG_FP64: fill_shot<double, double>(s); break;
+    case SG_FP16: fill_shot<_Float16, float>(s); break;
+    default: fill_shot<bf16, float>(s); break;
+    }
+    s->graph_valid = false;
+}
+
+void drop_graph(sg_solver *s)
+{
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+    s->graph_valid = false;
+}
+
+int upload_tables(sg_solver *s)
+{
+    const int n3[3] = {s->g.nx, s->g.ny, s->g.nz};
+    const double dt = s->cfg.dt;
+    std::vector<double> buf;
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> A(n3[a]), B(n3[a]);
+        for (int i = 0; i < n3[a]; ++i) {
+            const double hd = 0.5 * dt * s->damp[a][i];
+            A[i] = (1.0 - hd) / (1.0 + hd);
+            B[i] = 1.0 / (1.0 + hd);
+        }
+        buf.insert(buf.end(), A.begin(), A.end());
+        buf.insert(buf.end(), B.begin(), B.end());
+    }
+    if (compute_size(s->prec) == 8) {
+        SG_HIP(hipMemcpy(s->tables, buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(buf.begin(), buf.end());
+        SG_HIP(hipMemcpy(s->tables, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    return SG_OK;
+}
+
+int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
+{
+    const int acc = s->cfg.source_mode == SG_SOURCE_ACCUMULATE;
+    SG_HIP(launch_inject(s->prec, s->shot, s->amp_d, s->amp_acc, s->namp, acc, s->d_step, s->saved, s->stream));
+    if (e0) SG_HIP(hipEventRecord(e0, s->stream));
+    SG_HIP(launch_velocity(s->prec, s->cfg.kernel, s->shot, s->tile, s->stream));
+    if (e1) SG_HIP(hipEventRecord(e1, s->stream));
+    SG_HIP(launch_stress(s->prec, s->cfg.kernel, s->shot, s->tile, s->stream));
+    if (e2) SG_HIP(hipEventRecord(e2, s->stream));
+    if (s->nst > 0)
+        SG_HIP(launch_gather(s->prec, s->shot, s->st_idx, s->st_comp, s->nst, s->ncomp, s->interval, s->st_cap,
+                             s->st_out, s->d_step, s->stream));
+    return SG_OK;
+}
+
+int build_graph(sg_solver *s)
+{
+    drop_graph(s);
+    SG_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(s, nullptr, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc != SG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    SG_HIP(e);
+    s->graph = g;
+    SG_HIP(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    s->graph_valid = true;
+    return SG_OK;
+}
+
+void free_all(sg_solver *s)
+{
+    drop_graph(s);
+    for (auto ev : s->events) (void)hipEventDestroy(ev);
+    s->events.clear();
+    void *ptrs[] = {s->splits, s->tables, s->saved, s->d_step, s->amp_d, s->amp_acc,
+                    s->st_idx, s->st_comp, s->st_out, s->d_energy};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int f = 0; f < NFIELD; ++f)
+        if (s->fields[f]) (void)hipFree(s->fields[f]);
+    for (int m = 0; m < 3; ++m)
+        if (s->model[m]) (void)hipFree(s->model[m]);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+int check_solver(const sg_solver *s)
+{
+    if (!s) return fail(SG_ERR_INVALID, "solver handle is NULL");
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sg_api_version(void) { return SG_API_VERSION; }
+
+const char *sg_last_error(void) { return sg::last_error().c_str(); }
+
+int sg_create(const sg_config *cfg, sg_solver **out)
+{
+    if (!cfg || !out) return fail(SG_ERR_INVALID, "sg_create: NULL argument");
+    *out = nullptr;
+    const sg_config c = *cfg;
+    if (c.nx < 11 || c.ny < 11 || c.nz < 11) return fail(SG_ERR_INVALID, "sg_create: each extent must be >= 11");
+    if (c.pml < 0 || 2 * c.pml > c.nx - 10 || 2 * c.pml > c.ny - 10 || 2 * c.pml > c.nz - 10)
+        return fail(SG_ERR_INVALID, "sg_create: pml must satisfy 0 <= 2*pml <= n-10 on every axis");
+    if (!(c.dt > 0) || !(c.dx > 0) || !(c.dy > 0) || !(c.dz > 0))
+        return fail(SG_ERR_INVALID, "sg_create: dt, dx, dy, dz must be positive");
+    if (c.time_length < 0) return fail(SG_ERR_INVALID, "sg_create: time_length must be >= 0");
+    if (c.precision < SG_FP32 || c.precision > SG_BF16) return fail(SG_ERR_INVALID, "sg_create: bad precision");
+    if (c.source_mode != SG_SOURCE_TRANSIENT && c.source_mode != SG_SOURCE_ACCUMULATE)
+        return fail(SG_ERR_INVALID, "sg_create: bad source_mode");
+    if (c.kernel != SG_KERNEL_TILED && c.kernel != SG_KERNEL_SIMPLE)
+        return fail(SG_ERR_INVALID, "sg_create: bad kernel");
+    const long long N = (long long)c.nx * c.ny * c.nz;
+    if (N > (1LL << 31) - 1) return fail(SG_ERR_INVALID, "sg_create: grid exceeds 2^31-1 points per array");
+
+    sg_solver *s = new (std::nothrow) sg_solver();
+    if (!s) return fail(SG_ERR_NOMEM, "sg_create: host allocation failed");
+    s->cfg = c;
+    s->prec = c.precision;
+    for (int f = 0; f < NFIELD; ++f) s->fscale[f] = 1.0f;
+
+    Geom &g = s->g;
+    g.nx = c.nx;
+    g.ny = c.ny;
+    g.nz = c.nz;
+    g.pml = c.pml;
+    g.ux = c.nx - 2 * HALO;
+    g.uy = c.ny - 2 * HALO;
+    g.uz = c.nz - 2 * HALO;
+    g.sy = c.nx;
+    g.sz = (long long)c.nx * c.ny;
+    const long long P = c.pml;
+    g.zsec = 2 * P * g.uy * g.ux;
+    g.ysec = g.zsec + (g.uz - 2 * P) * 2 * P * g.ux;
+    g.nslots = g.ysec + (g.uz - 2 * P) * (long long)(g.uy - 2 * P) * 2 * P;
+
+    auto bail = [&](int rc) {
+        free_all(s);
+        delete s;
+        return rc;
+    };
+    int rc;
+#define SG_TRY(call)                      \
+    do {                                  \
+        hipError_t e_ = (call);           \
+        if (e_ != hipSuccess) {           \
+            rc = fail(e_ == hipErrorOutOfMemory ? SG_ERR_NOMEM : SG_ERR_HIP, \
+                      std::string("sg_create: ") + #call + ": " + hipGetErrorString(e_)); \
+            return bail(rc);              \
+        }                                 \
+    } while (0)
+
+    SG_TRY(hipGetDevice(&s->device));
+    SG_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    const size_t sT = storage_size(s->prec), sC = compute_size(s->prec);
+    for (int f = 0; f < NFIELD; ++f) {
+        SG_TRY(hipMalloc(&s->fields[f], N * sT));
+        SG_TRY(hipMemsetAsync(s->fields[f], 0, N * sT, s->stream));
+    }
+    for (int m = 0; m < 3; ++m) {
+        SG_TRY(hipMalloc(&s->model[m], N * sT));
+        SG_TRY(hipMemsetAsync(s->model[m], 0, N * sT, s->stream));
+    }
+    if (g.nslots > 0) {
+        SG_TRY(hipMalloc(&s->splits, (size_t)g.nslots * 24 * sC));
+        SG_TRY(hipMemsetAsync(s->splits, 0, (size_t)g.nslots * 24 * sC, s->stream));
+    }
+    SG_TRY(hipMalloc(&s->tables, 2 * (size_t)(c.nx + c.ny + c.nz) * sC));
+    SG_TRY(hipMalloc(&s->saved, 4 * sT));
+    SG_TRY(hipMemsetAsync(s->saved, 0, 4 * sT, s->stream));
+    SG_TRY(hipMalloc(&s->d_step, sizeof(int)));
+    SG_TRY(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
+    SG_TRY(hipMalloc(&s->d_energy, 2 * sizeof(double)));
+    SG_TRY(hipStreamSynchronize(s->stream));
+#undef SG_TRY
+
+    // default damping: the reference CPU formula (tau 3.4, V 2000, R 1e-3)
+    const int n3[3] = {c.nx, c.ny, c.nz};
+    const double h3[3] = {c.dx, c.dy, c.dz};
+    for (int a = 0; a < 3; ++a) {
+        s->damp[a].assign(n3[a], 0.0);
+        if (c.pml > 0) sg_damping_profile(n3[a], c.pml, h3[a], SG_DAMP_REFERENCE, 2000.0, 1e-3, 3.4, s->damp[a].data());
+    }
+    if ((rc = upload_tables(s)) != SG_OK) return bail(rc);
+    refresh_shot(s);
+    *out = s;
+    return SG_OK;
+}
+
+int sg_destroy(sg_solver *s)
+{
+    if (!s) return SG_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return SG_OK;
+}
+
+int sg_set_medium(sg_solver *s, const float *b, const float *lam, const float *mu)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!b || !lam || !mu) return fail(SG_ERR_INVALID, "sg_set_medium: NULL array");
+    const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz;
+    const float *src[3] = {b, lam, mu};
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < N; ++i) {
+        if (!(b[i] > 0) || !std::isfinite(b[i]) || !std::isfinite(lam[i]) || !(mu[i] > 0) || !std::isfinite(mu[i]))
+            return fail(SG_ERR_INVALID, "sg_set_medium: buoyancy and mu must be positive and finite, lambda finite "
+                                        "(first bad index " + std::to_string(i) + ")");
+    }
+    if (s->prec == SG_FP32) {
+        for (int m = 0; m < 3; ++m) SG_HIP(hipMemcpy(s->model[m], src[m], N * 4, hipMemcpyHostToDevice));
+    } else if (s->prec == SG_FP64) {
+        std::vector<double> d(N);
+        for (int m = 0; m < 3; ++m) {
+            for (size_t i = 0; i < N; ++i) d[i] = src[m][i];
+            SG_HIP(hipMemcpy(s->model[m], d.data(), N * 8, hipMemcpyHostToDevice));
+        }
+    } else {
+        return fail(SG_ERR_UNSUPPORTED, "sg_set_medium: FP16/BF16 storage is not available in this build");
+    }
+    return SG_OK;
+}
+
+int sg_set_damping(sg_solver *s, const double *dx, const double *dy, const double *dz)
+{
+    if (int rc = check_solver(s)) return rc;
+    const double *d3[3] = {dx, dy, dz};
+    const int n3[3] = {s->g.nx, s->g.ny, s->g.nz};
+    for (int a = 0; a < 3; ++a) {
+        if (!d3[a]) return fail(SG_ERR_INVALID, "sg_set_damping: NULL profile");
+        for (int i = 0; i < n3[a]; ++i)
+            if (!std::isfinite(d3[a][i]) || d3[a][i] < 0) return fail(SG_ERR_INVALID, "sg_set_damping: d must be finite and >= 0");
+    }
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    for (int a = 0; a < 3; ++a) s->damp[a].assign(d3[a], d3[a] + n3[a]);
+    return upload_tables(s);
+}
+
+int sg_set_source(sg_solver *s, int x, int y, int z, const float *wavelet, int n)
+{
+    if (int rc = check_solver(s)) return rc;
+    const Geom &g = s->g;
+    if (x < HALO || x >= g.nx - HALO || y < HALO || y >= g.ny - HALO || z < HALO || z >= g.nz - HALO)
+        return fail(SG_ERR_RANGE, "sg_set_source: source must lie in the updated region [5, n-5)");
+    if (n < 0 || (n > 0 && !wavelet)) return fail(SG_ERR_INVALID, "sg_set_source: bad wavelet");
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    if (s->amp_d) SG_HIP(hipFree(s->amp_d));
+    if (s->amp_acc) SG_HIP(hipFree(s->amp_acc));
+    s->amp_d = nullptr;
+    s->amp_acc = nullptr;
+    s->namp = n;
+    if (n > 0) {
+        // TRANSIENT: pow(dt, 2) * s(it) in double (ElasticWaveEquation3D.cpp:123)
+        std::vector<double> ad(n);
+        const double dt = (double)s->cfg.dt;
+        for (int i = 0; i < n; ++i) ad[i] = dt * dt * (double)wavelet[i];
+        SG_HIP(hipMalloc(&s->amp_d, n * sizeof(double)));
+        SG_HIP(hipMemcpy(s->amp_d, ad.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        // ACCUMULATE: deltaT * deltaT * sourceValue in the compute type (..._concise/.cu:216)
+        const size_t sC = compute_size(s->prec);
+        SG_HIP(hipMalloc(&s->amp_acc, n * sC));
+        if (sC == 8) {
+            std::vector<double> a(n);
+            for (int i = 0; i < n; ++i) a[i] = dt * dt * (double)wavelet[i];
+            SG_HIP(hipMemcpy(s->amp_acc, a.data(), n * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> a(n);
+            const float dtf = s->cfg.dt;
+            for (int i = 0; i < n; ++i) a[i] = dtf * dtf * wavelet[i];
+            SG_HIP(hipMemcpy(s->amp_acc, a.data(), n * 4, hipMemcpyHostToDevice));
+        }
+    }
+    s->src = x + (long long)y * g.sy + (long long)z * g.sz;
+    refresh_shot(s);
+    return SG_OK;
+}
+
+int sg_set_source_sample(sg_solver *s, int it, float value)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (it < 0 || it >= s->namp) return fail(SG_ERR_RANGE, "sg_set_source_sample: index outside the wavelet");
+    SG_HIP(hipSetDevice(s->device));
+    const double dt = (double)s->cfg.dt;
+    const double ad = dt * dt * (double)value;
+    // ordered after the steps already queued, before the ones queued later
+    SG_HIP(hipMemcpyAsync(s->amp_d + it, &ad, sizeof ad, hipMemcpyHostToDevice, s->stream));
+    if (compute_size(s->prec) == 8) {
+        SG_HIP(hipMemcpyAsync(static_cast<double *>(s->amp_acc) + it, &ad, 8, hipMemcpyHostToDevice, s->stream));
+    } else {
+        const float af = s->cfg.dt * s->cfg.dt * value;
+        SG_HIP(hipMemcpyAsync(static_cast<float *>(s->amp_acc) + it, &af, 4, hipMemcpyHostToDevice, s->stream));
+    }
+    SG_HIP(hipStreamSynchronize(s->stream));
+    return SG_OK;
+}
+
+int sg_set_stations(sg_solver *s, const int *xyz, int n, unsigned comp_mask, int interval)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (n < 0 || (n > 0 && !xyz) || interval < 1 || (comp_mask & ~0x1FFu))
+        return fail(SG_ERR_INVALID, "sg_set_stations: bad arguments");
+    if (n > 0 && (comp_mask & 0x1FFu) == 0) return fail(SG_ERR_INVALID, "sg_set_stations: empty component mask");
+    const Geom &g = s->g;
+    std::vector<long long> idx(n);
+    for (int i = 0; i < n; ++i) {
+        const int x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz)
+            return fail(SG_ERR_RANGE, "sg_set_stations: station " + std::to_string(i) + " outside the grid");
+        idx[i] = x + (long long)y * g.sy + (long long)z * g.sz;
+    }
+    std::vector<int> comp;
+    for (int f = 0; f < NFIELD; ++f)
+        if (comp_mask & (1u << f)) comp.push_back(f);
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    if (s->st_idx) SG_HIP(hipFree(s->st_idx));
+    if (s->st_comp) SG_HIP(hipFree(s->st_comp));
+    if (s->st_out) SG_HIP(hipFree(s->st_out));
+    s->st_idx = nullptr;
+    s->st_comp = nullptr;
+    s->st_out = nullptr;
+    s->nst = n;
+    s->ncomp = (int)comp.size();
+    s->interval = interval;
+    s->st_cap = 0;
+    if (n > 0) {
+        const long long steps = std::max(1, s->cfg.time_length);
+        s->st_cap = (steps + interval - 1) / interval;
+        SG_HIP(hipMalloc(&s->st_idx, n * sizeof(long long)));
+        SG_HIP(hipMemcpy(s->st_idx, idx.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+        SG_HIP(hipMalloc(&s->st_comp, comp.size() * sizeof(int)));
+        SG_HIP(hipMemcpy(s->st_comp, comp.data(), comp.size() * sizeof(int), hipMemcpyHostToDevice));
+        const size_t bytes = (size_t)s->st_cap * n * comp.size() * compute_size(s->prec);
+        SG_HIP(hipMalloc(&s->st_out, bytes));
+        SG_HIP(hipMemset(s->st_out, 0, bytes));
+    }
+    s->graph_valid = false;
+    return SG_OK;
+}
+
+int sg_step(sg_solver *s, int nsteps)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (nsteps < 0) return fail(SG_ERR_INVALID, "sg_step: nsteps < 0");
+    SG_HIP(hipSetDevice(s->device));
+    if (s->use_graph) {
+        if (!s->graph_valid)
+            if (int rc = build_graph(s)) return rc;
+        for (int i = 0; i < nsteps; ++i) SG_HIP(hipGraphLaunch(s->gexec, s->stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i)
+            if (int rc = enqueue_step(s, nullptr, nullptr, nullptr)) return rc;
+    }
+    s->steps_done += nsteps;
+    return SG_OK;
+}
+
+int sg_run(sg_solver *s)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (int rc = sg_step(s, s->cfg.time_length)) return rc;
+    return sg_synchronize(s);
+}
+
+int sg_synchronize(sg_solver *s)
+{
+    if (int rc = check_solver(s)) return rc;
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    return SG_OK;
+}
+
+int sg_reset(sg_solver *s)
+{
+    if (int rc = check_solver(s)) return rc;
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz, sT = storage_size(s->prec);
+    for (int f = 0; f < NFIELD; ++f) SG_HIP(hipMemsetAsync(s->fields[f], 0, N * sT, s->stream));
+    if (s->splits) SG_HIP(hipMemsetAsync(s->splits, 0, (size_t)s->g.nslots * 24 * compute_size(s->prec), s->stream));
+    SG_HIP(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
+    if (s->st_out)
+        SG_HIP(hipMemsetAsync(s->st_out, 0, (size_t)s->st_cap * s->nst * s->ncomp * compute_size(s->prec), s->stream));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    s->steps_done = 0;
+    return SG_OK;
+}
+
+long long sg_steps_done(const sg_solver *s) { return s ? s->steps_done : -1; }
+
+int sg_get_stations(sg_solver *s, float *out, long long count, long long *n_out)
+{
+    if (int rc = check_solver(s)) return rc;
+    const long long rows = std::min<long long>(s->st_cap, (s->steps_done + s->interval - 1) / s->interval);
+    const long long avail = rows * s->nst * s->ncomp;
+    if (n_out) *n_out = avail;
+    if (!out || count <= 0 || avail == 0) return SG_OK;
+    const long long n = std::min(count, avail);
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    if (compute_size(s->prec) == 4) {
+        SG_HIP(hipMemcpy(out, s->st_out, n * 4, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<double> d(n);
+        SG_HIP(hipMemcpy(d.data(), s->st_out, n * 8, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < n; ++i) out[i] = (float)d[i];
+    }
+    return SG_OK;
+}
+
+static int get_field_any(sg_solver *s, int field, float *outf, double *outd)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (field < 0 || field >= NFIELD) return fail(SG_ERR_RANGE, "sg_get_field: bad field id");
+    const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz;
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    if (s->prec == SG_FP32) {
+        if (outf) {
+            SG_HIP(hipMemcpy(outf, s->fields[field], N * 4, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> t(N);
+            SG_HIP(hipMemcpy(t.data(), s->fields[field], N * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; ++i) outd[i] = t[i];
+        }
+    } else if (s->prec == SG_FP64) {
+        if (outd) {
+            SG_HIP(hipMemcpy(outd, s->fields[field], N * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<double> t(N);
+            SG_HIP(hipMemcpy(t.data(), s->fields[field], N * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; ++i) outf[i] = (float)t[i];
+        }
+    } else {
+        return fail(SG_ERR_UNSUPPORTED, "sg_get_field: FP16/BF16 storage is not available in this build");
+    }
+    return SG_OK;
+}
+
+int sg_get_field(sg_solver *s, int field, float *out)
+{
+    if (!out) return fail(SG_ERR_INVALID, "sg_get_field: out is NULL");
+    return get_field_any(s, field, out, nullptr);
+}
+
+int sg_get_field_f64(sg_solver *s, int field, double *out)
+{
+    if (!out) return fail(SG_ERR_INVALID, "sg_get_field_f64: out is NULL");
+    return get_field_any(s, field, nullptr, out);
+}
+
+int sg_energy(sg_solver *s, double out[2])
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!out) return fail(SG_ERR_INVALID, "sg_energy: out is NULL");
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipMemsetAsync(s->d_energy, 0, 2 * sizeof(double), s->stream));
+    SG_HIP(launch_energy(s->prec, s->shot, s->d_energy, s->stream));
+    SG_HIP(hipMemcpyAsync(out, s->d_energy, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    return SG_OK;
+}
+
+int sg_step_timed(sg_solver *s, int nsteps, double *ms_vel, double *ms_str, double *ms_total)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (nsteps < 0) return fail(SG_ERR_INVALID, "sg_step_timed: nsteps < 0");
+    SG_HIP(hipSetDevice(s->device));
+    const int chunk = std::min(nsteps, 2048);
+    const size_t need = 3 * (size_t)chunk + 2;
+    while (s->events.size() < need) {
+        hipEvent_t e;
+        SG_HIP(hipEventCreate(&e));
+        s->events.push_back(e);
+    }
+    hipEvent_t t0 = s->events[3 * chunk], t1 = s->events[3 * chunk + 1];
+    double v = 0, st = 0;
+    SG_HIP(hipEventRecord(t0, s->stream));
+    for (int done = 0; done < nsteps;) {
+        const int m = std::min(chunk, nsteps - done);
+        for (int i = 0; i < m; ++i)
+            if (int rc = enqueue_step(s, s->events[3 * i], s->events[3 * i + 1], s->events[3 * i + 2])) return rc;
+        s->steps_done += m;
+        SG_HIP(hipEventSynchronize(s->events[3 * (m - 1) + 2]));
+        for (int i = 0; i < m; ++i) {
+            float a = 0, b = 0;
+            SG_HIP(hipEventElapsedTime(&a, s->events[3 * i], s->events[3 * i + 1]));
+            SG_HIP(hipEventElapsedTime(&b, s->events[3 * i + 1], s->events[3 * i + 2]));
+            v += a;
+            st += b;
+        }
+        done += m;
+    }
+    SG_HIP(hipEventRecord(t1, s->stream));
+    SG_HIP(hipEventSynchronize(t1));
+    float tot = 0;
+    SG_HIP(hipEventElapsedTime(&tot, t0, t1));
+    if (ms_vel) *ms_vel = v;
+    if (ms_str) *ms_str = st;
+    if (ms_total) *ms_total = tot;
+    return SG_OK;
+}
+
+int sg_set_graph(sg_solver *s, int enable)
+{
+    if (int rc = check_solver(s)) return rc;
+    s->use_graph = enable != 0;
+    if (!s->use_graph) drop_graph(s);
+    s->graph_valid = false;
+    return SG_OK;
+}
+
+int sg_set_tile(sg_solver *s, int tile_y, int z_chunk)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (tile_y != 4 && tile_y != 8 && tile_y != 16) return fail(SG_ERR_INVALID, "sg_set_tile: tile_y must be 4, 8 or 16");
+    if (z_chunk < 0) return fail(SG_ERR_INVALID, "sg_set_tile: z_chunk must be >= 0");
+    s->tile.ty = tile_y;
+    s->tile.zchunk = z_chunk;
+    s->graph_valid = false;
+    return SG_OK;
+}
+
+int sg_sweep_bytes(const sg_solver *s, double *vbytes, double *sbytes)
+{
+    if (!s) return fail(SG_ERR_INVALID, "solver handle is NULL");
+    const Geom &g = s->g;
+    const double sT = (double)storage_size(s->prec), sC = (double)compute_size(s->prec);
+    const double P2 = 2.0 * g.pml;
+    const double cnt[3][2] = {{g.ux - P2, P2}, {g.uy - P2, P2}, {g.uz - P2, P2}};
+    double vb = 0, sb = 0;
+    for (int m = 0; m < 8; ++m) {
+        const int dx = m & 1, dy = (m >> 1) & 1, dz = (m >> 2) & 1;
+        const double pts = cnt[0][dx] * cnt[1][dy] * cnt[2][dz];
+        if (pts <= 0) continue;
+        const int nd = dx + dy + dz;
+        // velocity: 6 stresses + buoyancy read, 3 components with 3 axes each
+        const double v = 7 * sT + 3 * comp_bytes(3, nd, sT, sC);
+        // stress: 3 velocities + lambda + mu read, 3 normal (3 axes), 3 shear (2 axes)
+        const double st = 5 * sT + 3 * comp_bytes(3, nd, sT, sC) + comp_bytes(2, dx + dy, sT, sC) +
+                          comp_bytes(2, dx + dz, sT, sC) + comp_bytes(2, dy + dz, sT, sC);
+        vb += pts * v;
+        sb += pts * st;
+    }
+    if (vbytes) *vbytes = vb;
+    if (sbytes) *sbytes = sb;
+    return SG_OK;
+}
+
+int sg_copy_bandwidth(long long bytes, int reps, double *gbps)
+{
+    if (bytes < 1024 || reps < 1 || !gbps) return fail(SG_ERR_INVALID, "sg_copy_bandwidth: bad arguments");
+    bytes &= ~15LL;
+    float *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SG_OK;
+    auto cleanup = [&]() {
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    };
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_fill_zero(a, bytes, st);
+    if (e == hipSuccess) e = launch_copy(a, b, bytes / 4, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(a, b, bytes / 4, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess)
+        rc = fail(SG_ERR_HIP, std::string("sg_copy_bandwidth: ") + hipGetErrorString(e));
+    else
+        *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    cleanup();
+    return rc;
+}
+
+}  // extern "C"
+
+// ---- accessors for the batch module ----------------------------------------
+namespace sg {
+const void *solver_shot(const sg_solver *s) { return s->shot; }
+int solver_prec(const sg_solver *s) { return s->prec; }
+hipStream_t solver_stream(const sg_solver *s) { return s->stream; }
+int solver_enqueue_aux_pre(sg_solver *s)
+{
+    const int acc = s->cfg.source_mode == SG_SOURCE_ACCUMULATE;
+    SG_HIP(launch_inject(s->prec, s->shot, s->amp_d, s->amp_acc, s->namp, acc, s->d_step, s->saved, s->stream));
+    return SG_OK;
+}
+int solver_enqueue_aux_post(sg_solver *s)
+{
+    if (s->nst > 0)
+        SG_HIP(launch_gather(s->prec, s->shot, s->st_idx, s->st_comp, s->nst, s->ncomp, s->interval, s->st_cap,
+                             s->st_out, s->d_step, s->stream));
+    s->steps_done += 1;
+    return SG_OK;
+}
+const Tile &solver_tile(const sg_solver *s) { return s->tile; }
+const Geom &solver_geom(const sg_solver *s) { return s->g; }
+}  // namespace sg

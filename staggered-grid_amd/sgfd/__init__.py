@@ -1,0 +1,359 @@
+"""Python host layer over the sgfd C ABI (libsgfd.so, include/sgfd.h).
+
+``Solver`` is a thin object wrapper of the C entry points; ``ElasticWaveEquation``
+mirrors the reference driver class (3DGPUwithPML_1DArraySingleGPU_concise/
+ElasticWaveEquation.h:17-60) so a reference user finds the same constructor,
+``run()``, ``propagate()``, ``sourceFunction()`` and ``stationDataOutput()``.
+
+All compute runs in the HIP library; if ``libsgfd.so`` is missing this module
+raises at import of the library (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from collections import namedtuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsgfd.so")
+
+FIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+PRECISIONS = {"fp32": 0, "fp64": 1, "fp16": 2, "bf16": 3}
+SOURCE_MODES = {"transient": 0, "accumulate": 1}
+KERNELS = {"tiled": 0, "simple": 1}
+DAMP_REFERENCE, DAMP_QUADRATIC = 0, 1
+RICKER_CEIL_DELAY, RICKER_NO_DELAY = 0, 1
+
+
+class SgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sgfd error {code}: {msg}")
+        self.code = code
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("pml", ctypes.c_int),
+        ("dt", ctypes.c_float), ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float),
+        ("diff_coef", ctypes.c_float * 5),
+        ("time_length", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("source_mode", ctypes.c_int),
+        ("kernel", ctypes.c_int),
+    ]
+
+
+_lib = None
+_FP = ctypes.POINTER(ctypes.c_float)
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_LLP = ctypes.POINTER(ctypes.c_longlong)
+_VP = ctypes.c_void_p
+
+# exported symbol -> (restype, argtypes); the list is the C ABI of include/sgfd.h
+API = {
+    "sg_api_version": (ctypes.c_int, []),
+    "sg_last_error": (ctypes.c_char_p, []),
+    "sg_create": (ctypes.c_int, [ctypes.POINTER(_Config), ctypes.POINTER(_VP)]),
+    "sg_destroy": (ctypes.c_int, [_VP]),
+    "sg_set_medium": (ctypes.c_int, [_VP, _FP, _FP, _FP]),
+    "sg_set_damping": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
+    "sg_set_source": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP, ctypes.c_int]),
+    "sg_set_source_sample": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_float]),
+    "sg_set_stations": (ctypes.c_int, [_VP, _IP, ctypes.c_int, ctypes.c_uint, ctypes.c_int]),
+    "sg_step": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_run": (ctypes.c_int, [_VP]),
+    "sg_synchronize": (ctypes.c_int, [_VP]),
+    "sg_reset": (ctypes.c_int, [_VP]),
+    "sg_steps_done": (ctypes.c_longlong, [_VP]),
+    "sg_get_stations": (ctypes.c_int, [_VP, _FP, ctypes.c_longlong, _LLP]),
+    "sg_get_field": (ctypes.c_int, [_VP, ctypes.c_int, _FP]),
+    "sg_get_field_f64": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
+    "sg_energy": (ctypes.c_int, [_VP, _DP]),
+    "sg_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP]),
+    "sg_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_set_tile": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "sg_sweep_bytes": (ctypes.c_int, [_VP, _DP, _DP]),
+    "sg_copy_bandwidth": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _DP]),
+    "sg_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.POINTER(_VP)]),
+    "sg_batch_step": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_batch_synchronize": (ctypes.c_int, [_VP]),
+    "sg_batch_destroy": (ctypes.c_int, [_VP]),
+    "sg_ricker": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, _FP]),
+    "sg_damping_profile": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, _DP]),
+    "sg_diff_coef": (ctypes.c_int, [ctypes.c_int, _FP]),
+    "sg_gaussian_field": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_ulonglong, ctypes.c_double, _FP]),
+}
+
+
+def lib():
+    """Load libsgfd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsgfd.so not built at {LIB_PATH}: run __graft_entry__.build() "
+                               "(make -C staggered-grid_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in API.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SgError(rc, lib().sg_last_error().decode())
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_FP)
+
+
+def diff_coef(kind: str = "reference") -> np.ndarray:
+    """Staggered coefficients: 'reference' literals or 'exact' rationals."""
+    out = np.zeros(5, np.float32)
+    _check(lib().sg_diff_coef({"reference": 0, "exact": 1}[kind], _fptr(out)))
+    return out
+
+
+def ricker(f0: float, dt: float, nt: int, delay: str = "ceil") -> np.ndarray:
+    out = np.zeros(nt, np.float32)
+    _check(lib().sg_ricker(f0, dt, nt, RICKER_CEIL_DELAY if delay == "ceil" else RICKER_NO_DELAY, _fptr(out)))
+    return out
+
+
+def damping_profile(n, pml, h, kind="reference", v=2000.0, r=1e-3, tau=3.4) -> np.ndarray:
+    out = np.zeros(n, np.float64)
+    k = DAMP_REFERENCE if kind == "reference" else DAMP_QUADRATIC
+    _check(lib().sg_damping_profile(n, pml, h, k, v, r, tau, out.ctypes.data_as(_DP)))
+    return out
+
+
+def gaussian_field(nx, ny, nz, corr, seed, clip=4.0) -> np.ndarray:
+    """Seeded unit-variance field with correlation exp(-r^2/corr^2), shape (nz, ny, nx)."""
+    out = np.zeros((nz, ny, nx), np.float32)
+    _check(lib().sg_gaussian_field(nx, ny, nz, float(corr), int(seed), float(clip), _fptr(out)))
+    return out
+
+
+def copy_bandwidth(nbytes: int, reps: int = 20) -> float:
+    g = ctypes.c_double()
+    _check(lib().sg_copy_bandwidth(int(nbytes), reps, ctypes.byref(g)))
+    return g.value
+
+
+class Solver:
+    """One shot on the current GPU.  Arrays are numpy, x-fastest, shape (nz, ny, nx)."""
+
+    def __init__(self, nx, ny, nz, pml, dt, dx, dy, dz, coef=None, time_length=0, precision="fp32",
+                 source_mode="transient", kernel="tiled"):
+        c = _Config()
+        c.nx, c.ny, c.nz, c.pml = nx, ny, nz, pml
+        c.dt, c.dx, c.dy, c.dz = dt, dx, dy, dz
+        coef = diff_coef("reference") if coef is None else np.asarray(coef, np.float32)
+        for i in range(5):
+            c.diff_coef[i] = float(coef[i])
+        c.time_length = time_length
+        c.precision = PRECISIONS[precision]
+        c.source_mode = SOURCE_MODES[source_mode]
+        c.kernel = KERNELS[kernel]
+        h = _VP()
+        _check(lib().sg_create(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self.shape = (nz, ny, nx)
+        self.nx, self.ny, self.nz, self.pml = nx, ny, nz, pml
+        self.dt = dt
+        self.time_length = time_length
+        self.precision = precision
+        self._nst = 0
+        self._ncomp = 0
+        self._interval = 1
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- inputs
+    def set_medium(self, b, lam, mu):
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(self.shape) for a in (b, lam, mu)]
+        _check(lib().sg_set_medium(self._h, *[_fptr(a) for a in arrs]))
+
+    def set_damping(self, dx, dy, dz):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (dx, dy, dz)]
+        assert arrs[0].size == self.nx and arrs[1].size == self.ny and arrs[2].size == self.nz
+        _check(lib().sg_set_damping(self._h, *[a.ctypes.data_as(_DP) for a in arrs]))
+
+    def set_source(self, x, y, z, wavelet):
+        w = np.ascontiguousarray(wavelet, np.float32)
+        _check(lib().sg_set_source(self._h, int(x), int(y), int(z), _fptr(w), w.size))
+
+    def set_source_sample(self, it, value):
+        _check(lib().sg_set_source_sample(self._h, int(it), float(value)))
+
+    def set_stations(self, xyz, comps=("vx", "vy", "vz"), interval=1):
+        st = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        mask = 0
+        for c in comps:
+            mask |= 1 << FIELD_ID[c]
+        _check(lib().sg_set_stations(self._h, st.ctypes.data_as(_IP), st.shape[0], mask, interval))
+        self._nst = st.shape[0]
+        self._ncomp = len([f for f in FIELDS if mask >> FIELD_ID[f] & 1])
+        self._interval = interval
+
+    # -- stepping
+    def step(self, n=1):
+        _check(lib().sg_step(self._h, int(n)))
+
+    def run(self):
+        _check(lib().sg_run(self._h))
+
+    def synchronize(self):
+        _check(lib().sg_synchronize(self._h))
+
+    def reset(self):
+        _check(lib().sg_reset(self._h))
+
+    @property
+    def steps_done(self):
+        return lib().sg_steps_done(self._h)
+
+    def step_timed(self, n):
+        v, s, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().sg_step_timed(self._h, int(n), ctypes.byref(v), ctypes.byref(s), ctypes.byref(t)))
+        return v.value, s.value, t.value
+
+    def set_graph(self, enable=True):
+        _check(lib().sg_set_graph(self._h, 1 if enable else 0))
+
+    def set_tile(self, ty, zchunk=0):
+        _check(lib().sg_set_tile(self._h, ty, zchunk))
+
+    def sweep_bytes(self):
+        v, s = ctypes.c_double(), ctypes.c_double()
+        _check(lib().sg_sweep_bytes(self._h, ctypes.byref(v), ctypes.byref(s)))
+        return v.value, s.value
+
+    # -- outputs
+    def stations(self) -> np.ndarray:
+        n = ctypes.c_longlong()
+        _check(lib().sg_get_stations(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        if n.value:
+            _check(lib().sg_get_stations(self._h, _fptr(out), n.value, ctypes.byref(n)))
+        return out.reshape(-1, self._nst, self._ncomp) if self._nst else out
+
+    def field(self, name, dtype=np.float32) -> np.ndarray:
+        fid = FIELD_ID[name] if isinstance(name, str) else int(name)
+        if dtype == np.float64:
+            out = np.zeros(self.shape, np.float64)
+            _check(lib().sg_get_field_f64(self._h, fid, out.ctypes.data_as(_DP)))
+        else:
+            out = np.zeros(self.shape, np.float32)
+            _check(lib().sg_get_field(self._h, fid, _fptr(out)))
+        return out
+
+    def fields(self, dtype=np.float32) -> np.ndarray:
+        return np.stack([self.field(f, dtype) for f in FIELDS])
+
+    def energy(self):
+        out = np.zeros(2, np.float64)
+        _check(lib().sg_energy(self._h, out.ctypes.data_as(_DP)))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Reference-shaped driver
+# ---------------------------------------------------------------------------
+
+GRID = namedtuple("GRID", "xLength yLength zLength")
+Delta = namedtuple("Delta", "t x y z")
+SOURCELOCATION = namedtuple("SOURCELOCATION", "x y z")
+STATIONLOCATION = namedtuple("STATIONLOCATION", "x y z")
+Medium = namedtuple("Medium", "buoyancy lambda_ mu")
+
+
+class ElasticWaveEquation:
+    """Mirror of ElasticWaveEquation (3DGPUwithPML_1DArraySingleGPU_concise/
+    ElasticWaveEquation.h:41-58): total-size GRID (PML and 5-cell halo
+    included), Delta (t, x, y, z), Medium (buoyancy, lambda, mu; x-fastest
+    flat or (nz, ny, nx)), SOURCELOCATION, one STATIONLOCATION or a list of
+    them, DifferenceCoefficient (5 floats) and the Ricker main frequency.
+
+    ``gpuDim`` is accepted and ignored (launch shapes are chosen by the
+    library).  Defaults follow the reference CUDA path: sxx recorded,
+    accumulating source, reference damping formula; pass
+    ``source_mode="transient"`` for the CPU path's behaviour.
+    """
+
+    def __init__(self, timeLength, gpuDim, pml, grid, delta, medCPU, sourceLocation, stationLocation, dc,
+                 mainFrequncy, components=("sxx",), source_mode="accumulate", precision="fp32",
+                 kernel="tiled", damping=None):
+        self.timeLength = timeLength
+        self.pml = pml
+        self.grid = GRID(*grid)
+        self.delta = Delta(*delta)
+        self.mainFrequncy = mainFrequncy
+        self.sourceLocationCPU = SOURCELOCATION(*sourceLocation)
+        st = np.asarray(stationLocation, np.int32).reshape(-1, 3)
+        self.solver = Solver(self.grid.xLength, self.grid.yLength, self.grid.zLength, pml, self.delta.t,
+                             self.delta.x, self.delta.y, self.delta.z, coef=dc, time_length=timeLength,
+                             precision=precision, source_mode=source_mode, kernel=kernel)
+        self.solver.set_medium(*medCPU)
+        if damping is not None:
+            self.solver.set_damping(*damping)
+        self._wavelet = np.array([self.sourceFunction(it) for it in range(timeLength)], np.float32)
+        self.solver.set_source(*self.sourceLocationCPU, self._wavelet)
+        self.solver.set_stations(st, comps=components)
+        self.stationDataCPU = None
+
+    def sourceFunction(self, timeIndex):
+        """Ricker of the reference (ElasticWaveEquation.cu:187-192), float arithmetic."""
+        return float(ricker(self.mainFrequncy, self.delta.t, timeIndex + 1)[timeIndex])
+
+    def propagate(self, timeIndex, sourceValue):
+        """One step with an explicit source sample (ElasticWaveEquation.cu:194-206)."""
+        if timeIndex != self.solver.steps_done:
+            raise ValueError("propagate() must be called with consecutive time indices")
+        self.solver.set_source_sample(timeIndex, sourceValue)
+        self.solver.step(1)
+
+    def run(self):
+        """All timeLength steps, then the station buffer to host (ElasticWaveEquation.cu:140-185)."""
+        self.solver.run()
+        self.stationDataCPU = self.solver.stations()
+        return self.stationDataCPU
+
+    def stationDataOutput(self, path="stationData.txt"):
+        """One value per line per recorded sample (ElasticWaveEquation.cu:1040-1050)."""
+        data = self.solver.stations() if self.stationDataCPU is None else self.stationDataCPU
+        with open(path, "w") as f:
+            for v in np.asarray(data).reshape(data.shape[0], -1):
+                f.write(" ".join(f"{x:g}" for x in v) + "\n")
+
+    def close(self):
+        self.solver.close()

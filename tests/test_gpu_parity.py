@@ -1,0 +1,233 @@
+"""Parity of the HIP solver (through the C ABI) with the CPU oracle.
+
+Tolerance: relative L2 <= 1e-5 for FP32 wavefields and station traces after
+the full run (the stated parity bar); FP64 against the FP64 oracle <= 1e-11.
+Both kernel variants (tiled, simple) are checked.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_golden
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+TOL64 = 1e-11
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def sgfd_mod():
+    import sgfd
+
+    sgfd.lib()
+    return sgfd
+
+
+def _solve(sgfd, p, b, lam, mu, wavelet, stations, comps, kernel="tiled", precision="fp32", damp=None,
+           source_mode="transient", graph=False, tile=None):
+    s = sgfd.Solver(p.nx, p.ny, p.nz, p.pml, p.dt, p.dx, p.dy, p.dz, coef=p.coef, time_length=p.nt,
+                    precision=precision, kernel=kernel, source_mode=source_mode)
+    try:
+        s.set_medium(b, lam, mu)
+        if damp is not None:
+            s.set_damping(*damp)
+        s.set_source(*p.src, wavelet)
+        s.set_stations(stations, comps=comps)
+        if graph:
+            s.set_graph(True)
+        if tile:
+            s.set_tile(*tile)
+        s.run()
+        return s.stations(), s.fields(np.float64 if precision == "fp64" else np.float32)
+    finally:
+        s.close()
+
+
+def _damp_for(O, p):
+    # the oracle holds V, R, tau and the spacings as float; use the same values
+    f = lambda x: float(np.float32(x))
+    return [O.damping_profile(n, p.pml, f(h), p.damp_mode, f(p.damp_v), f(p.damp_r), f(p.damp_tau), dtype=np.float64)
+            for n, h in ((p.nx, p.dx), (p.ny, p.dy), (p.nz, p.dz))]
+
+
+def _golden_case(O, name):
+    meta, g = load_golden(name)
+    NX, NY, NZ = meta["total"]
+    native = meta["kind"] == "native"
+    p = O.OracleParams(NX, NY, NZ, meta["pml"], meta["dt"], meta["dx"], meta["dy"], meta["dz"], meta["nt"],
+                       tuple(meta["src"]), coef=O.REF_COEF if native else O.EXACT_COEF,
+                       damp_mode=O.DAMP_REFERENCE if native else O.DAMP_QUADRATIC, damp_v=meta["damp_v"])
+    if native:
+        rho = np.full((NZ, NY, NX), 2500.0, np.float32)
+        V2 = np.float32(2000.0 * 2000.0)
+        b = (np.float32(1.0) / rho).astype(np.float32)
+        lam = (rho * V2).astype(np.float32)
+        mu = (rho * V2 / np.float32(3.0)).astype(np.float32)
+        w = O.ricker(meta["f0"], meta["dt"], meta["nt"], O.RICKER_CEIL_DELAY)
+    else:
+        b, lam, mu = g["b"], g["lam"], g["mu"]
+        w = O.ricker(meta["f0"], meta["dt"], meta["nt"], O.RICKER_NO_DELAY)
+    return meta, g, p, b, lam, mu, w
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "simple"])
+@pytest.mark.parametrize("name", ["g0_native_homog.npz", "g1_baseline_hetero.npz"])
+def test_golden_fp32(oracle_mod, sgfd_mod, name, kernel):
+    O = oracle_mod
+    meta, g, p, b, lam, mu, w = _golden_case(O, name)
+    _say(f"golden {name} kernel={kernel} grid={meta['total']} nt={meta['nt']}")
+    tr, F = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], sgfd_mod.FIELDS, kernel=kernel, damp=_damp_for(O, p))
+    # golden fields come from the compiled reference itself
+    assert _rel(F, g["fields"]) < TOL32
+    assert _rel(tr, g["traces"]) < TOL32
+    for f in range(9):
+        assert not F[f][:5].any() and not F[f][-5:].any()
+        assert not F[f][:, :5].any() and not F[f][:, -5:].any()
+        assert not F[f][:, :, :5].any() and not F[f][:, :, -5:].any()
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "simple"])
+def test_golden_fp64_matches_fp64_oracle(oracle_mod, sgfd_mod, kernel):
+    O = oracle_mod
+    meta, g, p, b, lam, mu, w = _golden_case(O, "g1_baseline_hetero.npz")
+    _say(f"fp64 g1 kernel={kernel}")
+    tr, F = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], sgfd_mod.FIELDS, kernel=kernel, precision="fp64",
+                   damp=_damp_for(O, p))
+    p64 = p
+    tr64, F64 = O.run(p64, b.astype(np.float64), lam.astype(np.float64), mu.astype(np.float64),
+                      w.astype(np.float64), g["stations"], dtype=np.float64)
+    assert _rel(F, F64) < TOL64
+    # the GPU station buffer is handed out as float32
+    assert _rel(tr, tr64) < 1e-6
+
+
+def _config1(O):
+    meta, g = load_golden("g2_config1_traces.npz")
+    NX, NY, NZ = meta["total"]
+    rho = np.full((NZ, NY, NX), meta["rho"], np.float64)
+    mu = (rho * meta["vs"] ** 2)
+    lam = (rho * meta["vp"] ** 2 - 2 * mu).astype(np.float32)
+    mu = mu.astype(np.float32)
+    b = (1 / rho).astype(np.float32)
+    p = O.OracleParams(NX, NY, NZ, meta["pml"], meta["dt"], meta["dx"], meta["dy"], meta["dz"], meta["nt"],
+                       tuple(meta["src"]), coef=O.EXACT_COEF, damp_mode=O.DAMP_QUADRATIC, damp_v=meta["damp_v"])
+    w = O.ricker(meta["f0"], meta["dt"], meta["nt"], O.RICKER_NO_DELAY)
+    return meta, g, p, b, lam, mu, w
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "simple"])
+def test_config1_full_run(oracle_mod, sgfd_mod, kernel):
+    """100^3 incl. a 10-cell PML, 500 steps: traces against the compiled
+    reference's golden vectors and the full final fields against the oracle."""
+    O = oracle_mod
+    meta, g, p, b, lam, mu, w = _config1(O)
+    _say(f"config1 100^3 500 steps kernel={kernel}")
+    tr, F = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], ("vx", "vy", "vz", "sxx"), kernel=kernel,
+                   damp=_damp_for(O, p))
+    assert _rel(tr, g["traces"]) < TOL32
+    for c in range(4):
+        assert _rel(tr[:, :, c], g["traces"][:, :, c]) < TOL32
+    assert _rel(F[:, 2::5, 2::5, 2::5], g["fields_sub"]) < TOL32
+    _say("config1 oracle full-field run")
+    _, Fo = O.run(p, b, lam, mu, w, None, fields=True)
+    for f in range(9):
+        assert _rel(F[f], Fo[f]) < TOL32, sgfd_mod.FIELDS[f]
+
+
+def test_reference_cuda_station_file(sgfd_mod):
+    """The reference's CUDA station file (..._concise/stationData.txt) before
+    any PML interaction, with the CUDA path's accumulating source."""
+    sgfd = sgfd_mod
+    ref = np.loadtxt(os.path.join(GOLDEN, "stationData_concise_gpu.txt"))
+    _say("reference CUDA station file, 300x200x100, 260 steps")
+    nx, ny, nz, pml, nt = 300, 200, 100, 10, 260
+    grid = sgfd.GRID(nx, ny, nz)
+    delta = sgfd.Delta(5e-4, 10.0, 10.0, 10.0)
+    rho = np.float32(2600.0)
+    lam0 = np.float32(2600.0 * 5000.0 ** 2)
+    shape = (nz, ny, nx)
+    med = (np.full(shape, np.float32(1.0) / rho, np.float32), np.full(shape, lam0, np.float32),
+           np.full(shape, np.float32(lam0 / np.float32(3.0)), np.float32))
+    ewq = sgfd.ElasticWaveEquation(nt, None, pml, grid, delta, med, (150, 100, 50), (160, 110, 60),
+                                   sgfd.diff_coef("reference"), 20.0)
+    try:
+        data = ewq.run()[:, 0, 0].astype(np.float64)
+    finally:
+        ewq.close()
+    n = 250
+    assert np.argmax(data[:n]) == np.argmax(ref[:n]) == 235
+    assert _rel(data[:n], ref[:n]) < 1e-4
+
+
+def test_graph_replay_matches_eager(oracle_mod, sgfd_mod):
+    O = oracle_mod
+    meta, g, p, b, lam, mu, w = _golden_case(O, "g1_baseline_hetero.npz")
+    _say("graph replay vs eager, g1")
+    d = _damp_for(O, p)
+    tr1, F1 = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], sgfd_mod.FIELDS, damp=d)
+    tr2, F2 = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], sgfd_mod.FIELDS, damp=d, graph=True)
+    assert np.array_equal(F1, F2)
+    assert np.array_equal(tr1, tr2)
+
+
+def test_energy_decays_after_source(oracle_mod, sgfd_mod):
+    sgfd = sgfd_mod
+    n, pml = 60, 10
+    _say("energy decay 60^3 pml 10")
+    rho = np.full((n, n, n), 2000.0)
+    mu = (rho * 1732.0 ** 2).astype(np.float32)
+    lam = (rho * 3000.0 ** 2).astype(np.float32) - 2 * mu
+    b = (1 / rho).astype(np.float32)
+    with sgfd.Solver(n, n, n, pml, 1e-3, 10.0, 10.0, 10.0, coef=sgfd.diff_coef("exact"), time_length=600) as s:
+        s.set_medium(b, lam, mu)
+        s.set_damping(*[sgfd.damping_profile(n, pml, 10.0, "quadratic", 3000.0)] * 3)
+        s.set_source(n // 2, n // 2, n // 2, sgfd.ricker(20.0, 1e-3, 600, "none"))
+        s.step(150)
+        e1 = s.energy()
+        s.step(450)
+        e2 = s.energy()
+    assert e1[0] > 0 and e2[0] < 0.05 * e1[0]
+    assert np.isfinite(e2).all()
+
+
+def test_step_timed_and_bytes(oracle_mod, sgfd_mod):
+    sgfd = sgfd_mod
+    _say("step_timed 64^3")
+    with sgfd.Solver(64, 64, 64, 8, 1e-3, 10.0, 10.0, 10.0, time_length=20) as s:
+        n = 64
+        s.set_medium(np.full(n ** 3, 1 / 2000.0, np.float32), np.full(n ** 3, 6e9, np.float32),
+                     np.full(n ** 3, 6e9, np.float32))
+        s.set_source(32, 32, 32, sgfd.ricker(15.0, 1e-3, 20))
+        v, st, tot = s.step_timed(20)
+        assert v > 0 and st > 0 and tot >= v + st
+        vb, sb = s.sweep_bytes()
+        u, P = 54, 8
+        interior = (u - 2 * P) ** 3
+        assert vb > 13 * 4 * u ** 3 and sb > 17 * 4 * u ** 3
+        assert vb < 28 * 4 * u ** 3
+        assert vb - 13 * 4 * interior > 0
+
+
+def test_invalid_inputs_raise(sgfd_mod):
+    sgfd = sgfd_mod
+    with sgfd.Solver(32, 32, 32, 4, 1e-3, 10.0, 10.0, 10.0, time_length=4) as s:
+        with pytest.raises(sgfd.SgError):
+            s.set_source(2, 16, 16, np.ones(4, np.float32))  # inside the halo
+        with pytest.raises(sgfd.SgError):
+            s.set_stations([[40, 0, 0]])
+        bad = np.ones(32 ** 3, np.float32)
+        bad[100] = -1
+        with pytest.raises(sgfd.SgError):
+            s.set_medium(bad, bad, bad)
