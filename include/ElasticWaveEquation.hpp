@@ -1,0 +1,133 @@
+// ElasticWaveEquation.hpp -- header-only C++ drop-in for the reference driver
+// class, built on the C ABI (sgfd.h).
+//
+// Mirrors 3DGPUwithPML_1DArraySingleGPU_concise/ElasticWaveEquation.h:17-60:
+// the same constructor arguments (GRID is the TOTAL extent, PML and 5-cell
+// halo included; Medium holds buoyancy/lambda/mu x-fastest; one station;
+// DifferenceCoefficient; main frequency), run(), propagate(it, s),
+// sourceFunction(it) and stationDataOutput().  GPUDim is accepted and
+// ignored: the library picks its own launch shape.  The default source mode
+// is the CUDA reference's (accumulating); pass SG_SOURCE_TRANSIENT for the
+// CPU reference's behaviour.  Errors throw sg::Error instead of exiting.
+#pragma once
+
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sgfd.h"
+
+typedef struct GRID {
+    int xLength, yLength, zLength;
+} GRID;
+typedef struct Delta {
+    float t, x, y, z;
+} Delta;
+typedef struct SOURCELOCATION {
+    int x, y, z;
+} SOURCELOCATION;
+typedef struct STATIONLOCATION {
+    int x, y, z;
+} STATIONLOCATION;
+typedef struct Medium {
+    float *buoyancy, *lambda, *mu;
+} Medium;
+typedef struct DifferenceCoefficient {
+    float diff_coef[5];
+} DifferenceCoefficient;
+typedef struct GPUDim {
+    unsigned blocksPerGrid[3], threadsPerBlock[3];
+} GPUDim;
+
+namespace sg {
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+inline void check(int rc)
+{
+    if (rc != SG_OK) throw Error(rc, sg_last_error());
+}
+}  // namespace sg
+
+class ElasticWaveEquation {
+public:
+    int timeLength;
+    GRID grid;
+    Delta delta;
+    int pml;
+    float mainFrequncy;
+    SOURCELOCATION sourceLocationCPU;
+    STATIONLOCATION stationLocationCPU;
+    std::vector<float> stationDataCPU;
+
+    ElasticWaveEquation(int timeLength, GPUDim /*gpuDim*/, int pml, GRID grid, Delta delta, Medium medCPU,
+                        SOURCELOCATION sourceLocation, STATIONLOCATION stationLocation, DifferenceCoefficient dc,
+                        float mainFrequncy, int source_mode = SG_SOURCE_ACCUMULATE, int precision = SG_FP32)
+        : timeLength(timeLength), grid(grid), delta(delta), pml(pml), mainFrequncy(mainFrequncy),
+          sourceLocationCPU(sourceLocation), stationLocationCPU(stationLocation)
+    {
+        sg_config c{};
+        c.nx = grid.xLength;
+        c.ny = grid.yLength;
+        c.nz = grid.zLength;
+        c.pml = pml;
+        c.dt = delta.t;
+        c.dx = delta.x;
+        c.dy = delta.y;
+        c.dz = delta.z;
+        for (int i = 0; i < 5; ++i) c.diff_coef[i] = dc.diff_coef[i];
+        c.time_length = timeLength;
+        c.precision = precision;
+        c.source_mode = source_mode;
+        c.kernel = SG_KERNEL_TILED;
+        sg::check(sg_create(&c, &s_));
+        sg::check(sg_set_medium(s_, medCPU.buoyancy, medCPU.lambda, medCPU.mu));
+        std::vector<float> w(timeLength);
+        for (int it = 0; it < timeLength; ++it) w[it] = sourceFunction(it);
+        sg::check(sg_set_source(s_, sourceLocation.x, sourceLocation.y, sourceLocation.z, w.data(), timeLength));
+        const int st[3] = {stationLocation.x, stationLocation.y, stationLocation.z};
+        sg::check(sg_set_stations(s_, st, 1, 1u << SG_SXX, 1));
+    }
+    ElasticWaveEquation(const ElasticWaveEquation &) = delete;
+    ElasticWaveEquation &operator=(const ElasticWaveEquation &) = delete;
+    ~ElasticWaveEquation() { sg_destroy(s_); }
+
+    // Ricker of the reference (..._concise/ElasticWaveEquation.cu:187-192)
+    float sourceFunction(int timeIndex)
+    {
+        std::vector<float> w(timeIndex + 1);
+        sg::check(sg_ricker(mainFrequncy, delta.t, timeIndex + 1, SG_RICKER_CEIL_DELAY, w.data()));
+        return w[timeIndex];
+    }
+
+    // one step with an explicit source value (..._concise/ElasticWaveEquation.cu:194-206)
+    void propagate(int timeIndex, float sourceValue)
+    {
+        sg::check(sg_set_source_sample(s_, timeIndex, sourceValue));
+        sg::check(sg_step(s_, 1));
+    }
+
+    // all steps, then the station buffer to the host (..._concise/ElasticWaveEquation.cu:140-185)
+    void run()
+    {
+        sg::check(sg_run(s_));
+        long long n = 0;
+        sg::check(sg_get_stations(s_, nullptr, 0, &n));
+        stationDataCPU.resize((size_t)n);
+        sg::check(sg_get_stations(s_, stationDataCPU.data(), n, &n));
+    }
+
+    // one value per line (..._concise/ElasticWaveEquation.cu:1040-1050)
+    void stationDataOutput(const char *path = "stationData.txt") const
+    {
+        std::ofstream f(path);
+        for (float v : stationDataCPU) f << v << '\n';
+    }
+
+    sg_solver *handle() { return s_; }
+
+private:
+    sg_solver *s_ = nullptr;
+};

@@ -135,6 +135,14 @@ int sg_set_source_sample(sg_solver *s, int it, float value);
  * [record][station][component].  Replaces any previous station set. */
 int sg_set_stations(sg_solver *s, const int *xyz, int n, unsigned comp_mask, int interval);
 
+/* FP16/BF16 storage: power-of-two scale per wavefield (stored = value *
+ * scale), ids as sg_field; default 1.  Must be set before the first step.
+ * The model arrays get their scales from sg_set_medium (max |x| mapped into
+ * [2^13, 2^14)).  sg_get_field_scales returns the 9 field scales followed by
+ * the buoyancy, lambda and mu scales. */
+int sg_set_field_scales(sg_solver *s, const float scales[9]);
+int sg_get_field_scales(const sg_solver *s, float out[12]);
+
 /* ---- stepping --------------------------------------------------------------- */
 
 /* Advances nsteps (source -> velocity -> stress -> record per step), queued on
@@ -173,6 +181,11 @@ int sg_set_graph(sg_solver *s, int enable);
 /* Tile shape of the tiled sweeps: y rows per block (4, 8 or 16) and z planes
  * per block (>= 1; 0 = whole updated depth). */
 int sg_set_tile(sg_solver *s, int tile_y, int z_chunk);
+/* Autotuner: times `steps` steps for each tile-row count (4, 8, 16) and
+ * z-chunk length (auto, 24, 48, 96, 192, whole depth), keeps the fastest
+ * (per-step sweep time in *best_ms) and RESETS the wavefields (sg_reset):
+ * call it before the real run.  The choice does not change any result. */
+int sg_autotune(sg_solver *s, int steps, int *best_ty, int *best_zc, double *best_ms);
 /* Algorithmic HBM bytes of one velocity and one stress sweep (the lean
  * split-PML word count times the storage element size). */
 int sg_sweep_bytes(const sg_solver *s, double *velocity_bytes, double *stress_bytes);
@@ -186,7 +199,13 @@ int sg_copy_bandwidth(long long bytes, int reps, double *gbps);
  * advances all of them over a cross-shot tile work-list.  The solvers stay
  * owned by the caller and must outlive the batch. */
 int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out);
+/* Launch shape of the batched sweeps: tile rows (4, 8, 16), z planes per work
+ * item, grid size (0 = one block per work item; smaller = persistent blocks
+ * striding over the list) and whether one step is replayed from a hipGraph. */
+int sg_batch_set_launch(sg_batch *b, int tile_y, int z_chunk, int grid, int use_graph);
 int sg_batch_step(sg_batch *b, int nsteps);
+/* sg_batch_step bracketed by HIP events on the batch stream; waits. */
+int sg_batch_step_timed(sg_batch *b, int nsteps, double *ms);
 int sg_batch_synchronize(sg_batch *b);
 int sg_batch_destroy(sg_batch *b);
 

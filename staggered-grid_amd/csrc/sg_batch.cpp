@@ -1,15 +1,243 @@
-// sg_batch.cpp -- batched shots (placeholder until the batched sweeps land).
+// sg_batch.cpp -- many independent shots advanced together.
+//
+// All shots' tiles form one work-list; each half-step is ONE launch of a
+// grid-stride (persistent) kernel over it, so small and uneven shots share
+// the GPU instead of running one under-filled launch each.  Source injection
+// and station recording of all shots are one launch each.  A step can be
+// captured in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
 #include "../../include/sgfd.h"
 #include "sg_error.hpp"
+#include "sg_internal.hpp"
+
+namespace sg {
+const void *solver_shot(const sg_solver *s);
+int solver_prec(const sg_solver *s);
+int solver_kernel(const sg_solver *s);
+hipStream_t solver_stream(const sg_solver *s);
+const Geom &solver_geom(const sg_solver *s);
+void solver_aux(const sg_solver *s, AuxDesc *a);
+void solver_add_steps(sg_solver *s, long long n);
+int solver_device(const sg_solver *s);
+}  // namespace sg
+
+using namespace sg;
+
+struct sg_batch {
+    std::vector<sg_solver *> shots;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    void *d_shots = nullptr;
+    AuxDesc *d_aux = nullptr;
+    WorkItem *d_items = nullptr;
+    int nitems = 0;   // padded to a multiple of 8
+    int nreal = 0;
+    int grid = 0;
+    int ty = 8;
+    int zc = 32;
+    int max_gather = 0;
+    bool use_graph = true;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+void drop_graph(sg_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+// Work-list: per shot, z-chunks of zc planes x y tiles x x tiles (x fastest);
+// then positions are interleaved so that item position p runs on XCD p % 8
+// (grid sizes are multiples of 8) and each XCD walks a contiguous band.
+int build_items(sg_batch *b)
+{
+    std::vector<WorkItem> seq;
+    for (size_t si = 0; si < b->shots.size(); ++si) {
+        const Geom &g = solver_geom(b->shots[si]);
+        const int nbx = (g.ux + 63) / 64, nby = (g.uy + b->ty - 1) / b->ty;
+        for (int k0 = HALO; k0 < g.nz - HALO; k0 += b->zc) {
+            const int k1 = std::min(k0 + b->zc, g.nz - HALO);
+            for (int by = 0; by < nby; ++by)
+                for (int bx = 0; bx < nbx; ++bx) seq.push_back(WorkItem{(int)si, bx, by, k0, k1});
+        }
+    }
+    b->nreal = (int)seq.size();
+    const int per = (b->nreal + 7) / 8;
+    b->nitems = per * 8;
+    std::vector<WorkItem> tab(b->nitems, WorkItem{-1, 0, 0, 0, 0});
+    for (int sidx = 0; sidx < b->nreal; ++sidx) {
+        const int x = sidx / per, local = sidx % per;
+        tab[(size_t)local * 8 + x] = seq[sidx];
+    }
+    if (b->d_items) SG_HIP(hipFree(b->d_items));
+    b->d_items = nullptr;
+    SG_HIP(hipMalloc(&b->d_items, tab.size() * sizeof(WorkItem)));
+    SG_HIP(hipMemcpy(b->d_items, tab.data(), tab.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    if (b->grid <= 0 || b->grid > b->nitems) b->grid = b->nitems;
+    b->grid = std::max(8, b->grid / 8 * 8);
+    b->graph_valid = false;
+    return SG_OK;
+}
+
+int enqueue(sg_batch *b)
+{
+    const int n = (int)b->shots.size();
+    SG_HIP(launch_inject_batch(b->d_shots, b->d_aux, n, b->stream));
+    SG_HIP(launch_velocity_batch(b->d_shots, b->d_items, b->nitems, b->grid, b->ty, b->stream));
+    SG_HIP(launch_stress_batch(b->d_shots, b->d_items, b->nitems, b->grid, b->ty, b->stream));
+    SG_HIP(launch_gather_batch(b->d_shots, b->d_aux, n, b->max_gather, b->stream));
+    return SG_OK;
+}
+
+void free_batch(sg_batch *b)
+{
+    drop_graph(b);
+    if (b->d_shots) (void)hipFree(b->d_shots);
+    if (b->d_aux) (void)hipFree(b->d_aux);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+}  // namespace
 
 extern "C" {
-int sg_batch_create(sg_solver *const *, int, sg_batch **out)
+
+int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out)
 {
-    if (out) *out = nullptr;
-    return sg::fail(SG_ERR_UNSUPPORTED, "sg_batch_create: not available in this build");
+    if (!out) return fail(SG_ERR_INVALID, "sg_batch_create: out is NULL");
+    *out = nullptr;
+    if (!shots || n < 1) return fail(SG_ERR_INVALID, "sg_batch_create: need at least one shot");
+    const int dev = solver_device(shots[0]);
+    for (int i = 0; i < n; ++i) {
+        if (!shots[i]) return fail(SG_ERR_INVALID, "sg_batch_create: NULL shot");
+        if (solver_prec(shots[i]) != SG_FP32) return fail(SG_ERR_UNSUPPORTED, "sg_batch_create: batching is FP32 only");
+        if (solver_device(shots[i]) != dev) return fail(SG_ERR_INVALID, "sg_batch_create: shots on different devices");
+    }
+    sg_batch *b = new (std::nothrow) sg_batch();
+    if (!b) return fail(SG_ERR_NOMEM, "sg_batch_create: host allocation failed");
+    b->shots.assign(shots, shots + n);
+    b->device = dev;
+    auto bail = [&](int rc) {
+        free_batch(b);
+        delete b;
+        return rc;
+    };
+    hipError_t e = hipSetDevice(dev);
+    std::vector<unsigned char> sh((size_t)n * sizeof(Shot<float, float>));
+    std::vector<AuxDesc> aux(n);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        e = hipStreamSynchronize(solver_stream(shots[i]));
+        std::memcpy(sh.data() + (size_t)i * sizeof(Shot<float, float>), solver_shot(shots[i]), sizeof(Shot<float, float>));
+        solver_aux(shots[i], &aux[i]);
+        b->max_gather = std::max(b->max_gather, aux[i].nst * aux[i].ncomp);
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&b->d_shots, sh.size());
+    if (e == hipSuccess) e = hipMemcpy(b->d_shots, sh.data(), sh.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->d_aux, n * sizeof(AuxDesc));
+    if (e == hipSuccess) e = hipMemcpy(b->d_aux, aux.data(), n * sizeof(AuxDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return bail(fail(e == hipErrorOutOfMemory ? SG_ERR_NOMEM : SG_ERR_HIP,
+                         std::string("sg_batch_create: ") + hipGetErrorString(e)));
+    if (int rc = build_items(b)) return bail(rc);
+    *out = b;
+    return SG_OK;
 }
-int sg_batch_step(sg_batch *, int) { return sg::fail(SG_ERR_UNSUPPORTED, "sg_batch_step: not available"); }
-int sg_batch_synchronize(sg_batch *) { return sg::fail(SG_ERR_UNSUPPORTED, "sg_batch_synchronize: not available"); }
-int sg_batch_destroy(sg_batch *) { return SG_OK; }
+
+int sg_batch_set_launch(sg_batch *b, int tile_y, int z_chunk, int grid, int use_graph)
+{
+    if (!b) return fail(SG_ERR_INVALID, "sg_batch_set_launch: NULL batch");
+    if (tile_y != 4 && tile_y != 8 && tile_y != 16) return fail(SG_ERR_INVALID, "sg_batch_set_launch: tile_y must be 4, 8 or 16");
+    if (z_chunk < 1 || grid < 0) return fail(SG_ERR_INVALID, "sg_batch_set_launch: bad z_chunk or grid");
+    SG_HIP(hipSetDevice(b->device));
+    SG_HIP(hipStreamSynchronize(b->stream));
+    b->ty = tile_y;
+    b->zc = z_chunk;
+    b->grid = grid;
+    b->use_graph = use_graph != 0;
+    drop_graph(b);
+    return build_items(b);
 }
+
+int sg_batch_step(sg_batch *b, int nsteps)
+{
+    if (!b) return fail(SG_ERR_INVALID, "sg_batch_step: NULL batch");
+    if (nsteps < 0) return fail(SG_ERR_INVALID, "sg_batch_step: nsteps < 0");
+    SG_HIP(hipSetDevice(b->device));
+    if (b->use_graph) {
+        if (!b->graph_valid) {
+            drop_graph(b);
+            SG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue(b);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            SG_HIP(e);
+            b->graph = g;
+            SG_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            b->graph_valid = true;
+        }
+        for (int i = 0; i < nsteps; ++i) SG_HIP(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i)
+            if (int rc = enqueue(b)) return rc;
+    }
+    for (auto *s : b->shots) solver_add_steps(s, nsteps);
+    return SG_OK;
+}
+
+int sg_batch_step_timed(sg_batch *b, int nsteps, double *ms)
+{
+    if (!b) return fail(SG_ERR_INVALID, "sg_batch_step_timed: NULL batch");
+    SG_HIP(hipSetDevice(b->device));
+    hipEvent_t e0, e1;
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    SG_HIP(hipEventRecord(e0, b->stream));
+    int rc = sg_batch_step(b, nsteps);
+    SG_HIP(hipEventRecord(e1, b->stream));
+    SG_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    SG_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+    return rc;
+}
+
+int sg_batch_synchronize(sg_batch *b)
+{
+    if (!b) return fail(SG_ERR_INVALID, "sg_batch_synchronize: NULL batch");
+    SG_HIP(hipSetDevice(b->device));
+    SG_HIP(hipStreamSynchronize(b->stream));
+    return SG_OK;
+}
+
+int sg_batch_destroy(sg_batch *b)
+{
+    if (!b) return SG_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_batch(b);
+    delete b;
+    return SG_OK;
+}
+
+}  // extern "C"

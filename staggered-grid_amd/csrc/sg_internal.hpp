@@ -91,8 +91,23 @@ struct Tile {
 
 // Work item of the batched launches: one (shot, x tile, y tile, z chunk).
 struct WorkItem {
-    int shot;
-    int i0, j0, k0, k1;
+    int shot;    // -1: padding entry
+    int bx, by;  // tile indices (x tiles of 64, y tiles of TY)
+    int k0, k1;  // z planes [k0, k1)
+};
+
+// Per-shot state of the small kernels (source, recording) for batched steps.
+struct AuxDesc {
+    const double *amp_d;
+    const float *amp_acc;
+    int namp, accumulate;
+    int *d_step;
+    float *saved;
+    const long long *st_idx;
+    const int *st_comp;
+    int nst, ncomp, interval;
+    long long cap;
+    float *st_out;
 };
 
 }  // namespace sg
@@ -116,4 +131,6 @@ hipError_t launch_velocity_batch(const void *shots_dev, const WorkItem *items, i
                                  int ty, hipStream_t st);
 hipError_t launch_stress_batch(const void *shots_dev, const WorkItem *items, int nitems, int grid,
                                int ty, hipStream_t st);
+hipError_t launch_inject_batch(const void *shots_dev, const AuxDesc *aux, int nshots, hipStream_t st);
+hipError_t launch_gather_batch(const void *shots_dev, const AuxDesc *aux, int nshots, int max_n, hipStream_t st);
 }  // namespace sg

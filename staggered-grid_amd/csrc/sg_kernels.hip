@@ -253,6 +253,43 @@ __global__ void k_gather(const Shot<T, C> s, const long long *idx, const int *co
     out[row * (long long)nst * ncomp + t] = Cvt<T, C>::ld(s.f[f][idx[st]], s.finv[f]);
 }
 
+// batched FP32 forms of the two kernels above: one shot per thread / per grid row
+__global__ void k_inject_batch(const Shot<float, float> *shots, const AuxDesc *aux, int nshots)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nshots) return;
+    const Shot<float, float> &s = shots[t];
+    const AuxDesc &a = aux[t];
+    const int it = *a.d_step;
+    *a.d_step = it + 1;
+    if (s.src < 0) return;
+    for (int c = 0; c < 3; ++c) {
+        float *p = s.f[SXX + c] + s.src;
+        if (a.accumulate) {
+            if (it < a.namp) *p = *p + a.amp_acc[it];
+        } else {
+            a.saved[c] = *p;
+            if (it < a.namp) *p = (float)((double)*p + a.amp_d[it]);
+        }
+    }
+}
+
+__global__ void k_gather_batch(const Shot<float, float> *shots, const AuxDesc *aux, int nshots)
+{
+    const int sh = blockIdx.y;
+    if (sh >= nshots) return;
+    const AuxDesc &a = aux[sh];
+    const int n = a.nst * a.ncomp;
+    const int it = *a.d_step - 1;
+    if (n == 0 || it < 0 || it % a.interval) return;
+    const long long row = it / a.interval;
+    if (row >= a.cap) return;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int st = t / a.ncomp, q = t - st * a.ncomp;
+        a.st_out[row * (long long)n + t] = shots[sh].f[a.st_comp[q]][a.st_idx[st]];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FP64 energy over the updated region
 // ---------------------------------------------------------------------------
@@ -416,6 +453,21 @@ static hipError_t energy_t(const void *shot, double *out2, hipStream_t st)
 hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st)
 {
     SG_DISPATCH(prec, energy_t, shot, out2, st)
+}
+
+hipError_t launch_inject_batch(const void *shots_dev, const AuxDesc *aux, int nshots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_inject_batch, dim3((nshots + 63) / 64), dim3(64), 0, st,
+                       static_cast<const Shot<float, float> *>(shots_dev), aux, nshots);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_batch(const void *shots_dev, const AuxDesc *aux, int nshots, int max_n, hipStream_t st)
+{
+    if (max_n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_batch, dim3((max_n + 255) / 256, nshots), dim3(256), 0, st,
+                       static_cast<const Shot<float, float> *>(shots_dev), aux, nshots);
+    return hipGetLastError();
 }
 
 hipError_t launch_copy(const float *src, float *dst, long long n, hipStream_t st)

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -44,6 +45,44 @@ double comp_bytes(int naxes, int ndamped, double sT, double sC)
 {
     if (ndamped == naxes) return 2.0 * ndamped * sC + sT;
     return 2.0 * sT + 2.0 * ndamped * sC;
+}
+
+// float -> 16-bit storage, round to nearest even
+uint16_t to_half_bits(float v)
+{
+    _Float16 h = (_Float16)v;
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+float from_half_bits(uint16_t b)
+{
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return (float)h;
+}
+uint16_t to_bf16_bits(float v)
+{
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float from_bf16_bits(uint16_t b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+// power-of-two scale that maps max|x| into [2^13, 2^14) (FP16 max is 65504)
+float pow2_scale(double maxabs)
+{
+    if (!(maxabs > 0) || !std::isfinite(maxabs)) return 1.0f;
+    const int e = (int)std::floor(std::log2(16384.0 / maxabs));
+    return std::ldexp(1.0f, std::max(-126, std::min(127, e)));
 }
 
 }  // namespace
@@ -363,8 +402,47 @@ int sg_set_medium(sg_solver *s, const float *b, const float *lam, const float *m
             SG_HIP(hipMemcpy(s->model[m], d.data(), N * 8, hipMemcpyHostToDevice));
         }
     } else {
-        return fail(SG_ERR_UNSUPPORTED, "sg_set_medium: FP16/BF16 storage is not available in this build");
+        // 16-bit storage: one power-of-two scale per model array
+        float *sc[3] = {&s->bscale, &s->lscale, &s->mscale};
+        std::vector<uint16_t> h(N);
+        for (int m = 0; m < 3; ++m) {
+            double mx = 0;
+            for (size_t i = 0; i < N; ++i) mx = std::max(mx, (double)std::fabs(src[m][i]));
+            *sc[m] = pow2_scale(mx);
+            for (size_t i = 0; i < N; ++i)
+                h[i] = s->prec == SG_FP16 ? to_half_bits(src[m][i] * *sc[m]) : to_bf16_bits(src[m][i] * *sc[m]);
+            SG_HIP(hipMemcpy(s->model[m], h.data(), N * 2, hipMemcpyHostToDevice));
+        }
+        refresh_shot(s);
     }
+    return SG_OK;
+}
+
+int sg_set_field_scales(sg_solver *s, const float scales[9])
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!scales) return fail(SG_ERR_INVALID, "sg_set_field_scales: NULL");
+    for (int f = 0; f < NFIELD; ++f) {
+        const float v = scales[f];
+        int e;
+        if (!(v > 0) || !std::isfinite(v) || std::frexp(v, &e) != 0.5f)
+            return fail(SG_ERR_INVALID, "sg_set_field_scales: scales must be positive powers of two");
+    }
+    if (s->steps_done != 0) return fail(SG_ERR_STATE, "sg_set_field_scales: set scales before stepping (or after sg_reset)");
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    for (int f = 0; f < NFIELD; ++f) s->fscale[f] = s->prec == SG_FP16 || s->prec == SG_BF16 ? scales[f] : 1.0f;
+    refresh_shot(s);
+    return SG_OK;
+}
+
+int sg_get_field_scales(const sg_solver *s, float out[12])
+{
+    if (!s || !out) return fail(SG_ERR_INVALID, "sg_get_field_scales: NULL");
+    for (int f = 0; f < NFIELD; ++f) out[f] = s->fscale[f];
+    out[9] = s->bscale;
+    out[10] = s->lscale;
+    out[11] = s->mscale;
     return SG_OK;
 }
 
@@ -581,7 +659,14 @@ static int get_field_any(sg_solver *s, int field, float *outf, double *outd)
             for (size_t i = 0; i < N; ++i) outf[i] = (float)t[i];
         }
     } else {
-        return fail(SG_ERR_UNSUPPORTED, "sg_get_field: FP16/BF16 storage is not available in this build");
+        std::vector<uint16_t> t(N);
+        SG_HIP(hipMemcpy(t.data(), s->fields[field], N * 2, hipMemcpyDeviceToHost));
+        const float inv = 1.0f / s->fscale[field];
+        for (size_t i = 0; i < N; ++i) {
+            const float v = (s->prec == SG_FP16 ? from_half_bits(t[i]) : from_bf16_bits(t[i])) * inv;
+            if (outf) outf[i] = v;
+            else outd[i] = v;
+        }
     }
     return SG_OK;
 }
@@ -670,6 +755,47 @@ int sg_set_tile(sg_solver *s, int tile_y, int z_chunk)
     return SG_OK;
 }
 
+int sg_autotune(sg_solver *s, int steps, int *best_ty, int *best_zc, double *best_ms)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (steps < 1) return fail(SG_ERR_INVALID, "sg_autotune: steps < 1");
+    if (s->cfg.kernel != SG_KERNEL_TILED) return fail(SG_ERR_STATE, "sg_autotune: the solver uses the per-point kernels");
+    // Tile rows x z-chunk length; each candidate runs `steps` timed steps
+    // from the current state, the fastest wins, then the state is reset.
+    const int uz = s->g.uz;
+    const int zcs[] = {0, 24, 48, 96, 192, uz};
+    const int tys[] = {4, 8, 16};
+    double best = 1e300;
+    int bty = s->tile.ty, bzc = s->tile.zchunk;
+    const bool graph = s->use_graph;
+    s->use_graph = false;
+    for (int ty : tys) {
+        for (int zc : zcs) {
+            if (zc > uz) continue;
+            s->tile.ty = ty;
+            s->tile.zchunk = zc;
+            double v = 0, st = 0, tot = 0;
+            if (int rc = sg_step_timed(s, steps, &v, &st, &tot)) {
+                s->use_graph = graph;
+                return rc;
+            }
+            if (v + st < best) {
+                best = v + st;
+                bty = ty;
+                bzc = zc;
+            }
+        }
+    }
+    s->use_graph = graph;
+    s->tile.ty = bty;
+    s->tile.zchunk = bzc;
+    s->graph_valid = false;
+    if (best_ty) *best_ty = bty;
+    if (best_zc) *best_zc = bzc;
+    if (best_ms) *best_ms = best / steps;
+    return sg_reset(s);
+}
+
 int sg_sweep_bytes(const sg_solver *s, double *vbytes, double *sbytes)
 {
     if (!s) return fail(SG_ERR_INVALID, "solver handle is NULL");
@@ -738,21 +864,25 @@ int sg_copy_bandwidth(long long bytes, int reps, double *gbps)
 namespace sg {
 const void *solver_shot(const sg_solver *s) { return s->shot; }
 int solver_prec(const sg_solver *s) { return s->prec; }
+int solver_kernel(const sg_solver *s) { return s->cfg.kernel; }
+int solver_device(const sg_solver *s) { return s->device; }
 hipStream_t solver_stream(const sg_solver *s) { return s->stream; }
-int solver_enqueue_aux_pre(sg_solver *s)
-{
-    const int acc = s->cfg.source_mode == SG_SOURCE_ACCUMULATE;
-    SG_HIP(launch_inject(s->prec, s->shot, s->amp_d, s->amp_acc, s->namp, acc, s->d_step, s->saved, s->stream));
-    return SG_OK;
-}
-int solver_enqueue_aux_post(sg_solver *s)
-{
-    if (s->nst > 0)
-        SG_HIP(launch_gather(s->prec, s->shot, s->st_idx, s->st_comp, s->nst, s->ncomp, s->interval, s->st_cap,
-                             s->st_out, s->d_step, s->stream));
-    s->steps_done += 1;
-    return SG_OK;
-}
-const Tile &solver_tile(const sg_solver *s) { return s->tile; }
 const Geom &solver_geom(const sg_solver *s) { return s->g; }
+void solver_add_steps(sg_solver *s, long long n) { s->steps_done += n; }
+void solver_aux(const sg_solver *s, AuxDesc *a)
+{
+    a->amp_d = s->amp_d;
+    a->amp_acc = static_cast<const float *>(s->amp_acc);
+    a->namp = s->namp;
+    a->accumulate = s->cfg.source_mode == SG_SOURCE_ACCUMULATE;
+    a->d_step = s->d_step;
+    a->saved = static_cast<float *>(s->saved);
+    a->st_idx = s->st_idx;
+    a->st_comp = s->st_comp;
+    a->nst = s->nst;
+    a->ncomp = s->ncomp;
+    a->interval = s->interval;
+    a->cap = s->st_cap;
+    a->st_out = static_cast<float *>(s->st_out);
+}
 }  // namespace sg

@@ -88,25 +88,41 @@ __device__ __forceinline__ C qbwd(const C *q, int c, const C *k)
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
+template <class C, int NY>
+struct VelLds {
+    static constexpr int TY = 4 * NY, YH = TY + 10;
+    C sxx[TY][XW];
+    C sxz[TY][XW];
+    C syy[YH][TX];
+    C syz[YH][TX];
+    C sxy[YH][XW];
+    C b[TY + 1][TX + 1];
+};
+
+template <class C, int NY>
+struct StrLds {
+    static constexpr int TY = 4 * NY, YH = TY + 10;
+    C vx[YH][XW];
+    C vy[YH][XW];
+    C vz[YH][XW];
+    C rm[2][TY + 1][TX + 1];  // 1/mu, planes k and k+1 (ring)
+};
+
+// One velocity tile: columns [i0, i0+64) x rows [j0, j0+TY), planes [k0, k1).
 template <class T, class C, int NY>
-__global__ __launch_bounds__(NTHR) void k_velocity_tiled(const Shot<T, C> s, const TileMap m)
+__device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int by, int k0, int k1,
+                                              VelLds<C, NY> &L)
 {
     constexpr int TY = 4 * NY;
-    constexpr int YH = TY + 10;
-    __shared__ C l_sxx[TY][XW];
-    __shared__ C l_sxz[TY][XW];
-    __shared__ C l_syy[YH][TX];
-    __shared__ C l_syz[YH][TX];
-    __shared__ C l_sxy[YH][XW];
-    __shared__ C l_b[TY + 1][TX + 1];
-
-    int bx, by, bz;
-    if (!decode_block(m, bx, by, bz)) return;
+    auto &l_sxx = L.sxx;
+    auto &l_sxz = L.sxz;
+    auto &l_syy = L.syy;
+    auto &l_syz = L.syz;
+    auto &l_sxy = L.sxy;
+    auto &l_b = L.b;
     const Geom &g = s.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = HALO + bx * TX, j0 = HALO + by * TY;
-    const int k0 = HALO + bz * m.zc;
-    const int k1 = min(k0 + m.zc, g.nz - HALO);
     const int i = i0 + tx;
     const long long sy = g.sy, sz = g.sz;
     const bool xin = i < g.nx;              // inside the array (loads)
@@ -273,26 +289,31 @@ __global__ __launch_bounds__(NTHR) void k_velocity_tiled(const Shot<T, C> s, con
     }
 }
 
+template <class T, class C, int NY>
+__global__ __launch_bounds__(NTHR) void k_velocity_tiled(const Shot<T, C> s, const TileMap m)
+{
+    __shared__ VelLds<C, NY> L;
+    int bx, by, bz;
+    if (!decode_block(m, bx, by, bz)) return;
+    const int k0 = HALO + bz * m.zc;
+    velocity_tile<T, C, NY>(s, bx, by, k0, min(k0 + m.zc, s.g.nz - HALO), L);
+}
+
 // ---------------------------------------------------------------------------
 // stress sweep
 // ---------------------------------------------------------------------------
 template <class T, class C, int NY>
-__global__ __launch_bounds__(NTHR) void k_stress_tiled(const Shot<T, C> s, const TileMap m)
+__device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by, int k0, int k1,
+                                            StrLds<C, NY> &L)
 {
     constexpr int TY = 4 * NY;
-    constexpr int YH = TY + 10;
-    __shared__ C l_vx[YH][XW];
-    __shared__ C l_vy[YH][XW];
-    __shared__ C l_vz[YH][XW];
-    __shared__ C l_rm[2][TY + 1][TX + 1];  // 1/mu, planes k and k+1 (ring)
-
-    int bx, by, bz;
-    if (!decode_block(m, bx, by, bz)) return;
+    auto &l_vx = L.vx;
+    auto &l_vy = L.vy;
+    auto &l_vz = L.vz;
+    auto &l_rm = L.rm;
     const Geom &g = s.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = HALO + bx * TX, j0 = HALO + by * TY;
-    const int k0 = HALO + bz * m.zc;
-    const int k1 = min(k0 + m.zc, g.nz - HALO);
     const int i = i0 + tx;
     const long long sy = g.sy, sz = g.sz;
     const bool xin = i < g.nx;
@@ -503,6 +524,47 @@ __global__ __launch_bounds__(NTHR) void k_stress_tiled(const Shot<T, C> s, const
     }
 }
 
+template <class T, class C, int NY>
+__global__ __launch_bounds__(NTHR) void k_stress_tiled(const Shot<T, C> s, const TileMap m)
+{
+    __shared__ StrLds<C, NY> L;
+    int bx, by, bz;
+    if (!decode_block(m, bx, by, bz)) return;
+    const int k0 = HALO + bz * m.zc;
+    stress_tile<T, C, NY>(s, bx, by, k0, min(k0 + m.zc, s.g.nz - HALO), L);
+}
+
+// ---------------------------------------------------------------------------
+// batched sweeps: a grid-stride loop over a cross-shot work-list.  Item
+// positions are arranged on the host so that position p runs on XCD p % 8
+// and each XCD sees a contiguous band of one shot's tiles.
+// ---------------------------------------------------------------------------
+template <int NY>
+__global__ __launch_bounds__(NTHR) void k_velocity_batch(const Shot<float, float> *__restrict__ shots,
+                                                         const WorkItem *__restrict__ items, int nitems)
+{
+    __shared__ VelLds<float, NY> L;
+    for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
+        const WorkItem w = items[p];
+        if (w.shot < 0) continue;
+        velocity_tile<float, float, NY>(shots[w.shot], w.bx, w.by, w.k0, w.k1, L);
+        __syncthreads();
+    }
+}
+
+template <int NY>
+__global__ __launch_bounds__(NTHR) void k_stress_batch(const Shot<float, float> *__restrict__ shots,
+                                                       const WorkItem *__restrict__ items, int nitems)
+{
+    __shared__ StrLds<float, NY> L;
+    for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
+        const WorkItem w = items[p];
+        if (w.shot < 0) continue;
+        stress_tile<float, float, NY>(shots[w.shot], w.bx, w.by, w.k0, w.k1, L);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -556,6 +618,28 @@ hipError_t stress_tiled(const Shot<T, C> &s, const Tile &t, hipStream_t st)
     if (ty == 4) hipLaunchKernelGGL((k_stress_tiled<T, C, 1>), grid, block, 0, st, s, m);
     else if (ty == 8) hipLaunchKernelGGL((k_stress_tiled<T, C, 2>), grid, block, 0, st, s, m);
     else hipLaunchKernelGGL((k_stress_tiled<T, C, 4>), grid, block, 0, st, s, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity_batch(const void *shots_dev, const WorkItem *items, int nitems, int grid, int ty,
+                                 hipStream_t st)
+{
+    const Shot<float, float> *sh = static_cast<const Shot<float, float> *>(shots_dev);
+    const dim3 block(TX, 4);
+    if (ty == 4) hipLaunchKernelGGL((k_velocity_batch<1>), dim3(grid), block, 0, st, sh, items, nitems);
+    else if (ty == 8) hipLaunchKernelGGL((k_velocity_batch<2>), dim3(grid), block, 0, st, sh, items, nitems);
+    else hipLaunchKernelGGL((k_velocity_batch<4>), dim3(grid), block, 0, st, sh, items, nitems);
+    return hipGetLastError();
+}
+
+hipError_t launch_stress_batch(const void *shots_dev, const WorkItem *items, int nitems, int grid, int ty,
+                               hipStream_t st)
+{
+    const Shot<float, float> *sh = static_cast<const Shot<float, float> *>(shots_dev);
+    const dim3 block(TX, 4);
+    if (ty == 4) hipLaunchKernelGGL((k_stress_batch<1>), dim3(grid), block, 0, st, sh, items, nitems);
+    else if (ty == 8) hipLaunchKernelGGL((k_stress_batch<2>), dim3(grid), block, 0, st, sh, items, nitems);
+    else hipLaunchKernelGGL((k_stress_batch<4>), dim3(grid), block, 0, st, sh, items, nitems);
     return hipGetLastError();
 }
 

@@ -64,6 +64,8 @@ API = {
     "sg_set_source": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP, ctypes.c_int]),
     "sg_set_source_sample": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_float]),
     "sg_set_stations": (ctypes.c_int, [_VP, _IP, ctypes.c_int, ctypes.c_uint, ctypes.c_int]),
+    "sg_set_field_scales": (ctypes.c_int, [_VP, _FP]),
+    "sg_get_field_scales": (ctypes.c_int, [_VP, _FP]),
     "sg_step": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_run": (ctypes.c_int, [_VP]),
     "sg_synchronize": (ctypes.c_int, [_VP]),
@@ -77,9 +79,12 @@ API = {
     "sg_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_set_tile": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "sg_sweep_bytes": (ctypes.c_int, [_VP, _DP, _DP]),
+    "sg_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _IP, _DP]),
     "sg_copy_bandwidth": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _DP]),
     "sg_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.POINTER(_VP)]),
+    "sg_batch_set_launch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "sg_batch_step": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_batch_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     "sg_batch_synchronize": (ctypes.c_int, [_VP]),
     "sg_batch_destroy": (ctypes.c_int, [_VP]),
     "sg_ricker": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, _FP]),
@@ -225,6 +230,16 @@ class Solver:
         self._ncomp = len([f for f in FIELDS if mask >> FIELD_ID[f] & 1])
         self._interval = interval
 
+    def set_field_scales(self, scales):
+        sc = np.ascontiguousarray(scales, np.float32)
+        assert sc.size == 9
+        _check(lib().sg_set_field_scales(self._h, _fptr(sc)))
+
+    def field_scales(self):
+        out = np.zeros(12, np.float32)
+        _check(lib().sg_get_field_scales(self._h, _fptr(out)))
+        return out
+
     # -- stepping
     def step(self, n=1):
         _check(lib().sg_step(self._h, int(n)))
@@ -252,6 +267,12 @@ class Solver:
 
     def set_tile(self, ty, zchunk=0):
         _check(lib().sg_set_tile(self._h, ty, zchunk))
+
+    def autotune(self, steps=5):
+        """Pick the fastest tile shape (resets the wavefields)."""
+        ty, zc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().sg_autotune(self._h, steps, ctypes.byref(ty), ctypes.byref(zc), ctypes.byref(ms)))
+        return ty.value, zc.value, ms.value
 
     def sweep_bytes(self):
         v, s = ctypes.c_double(), ctypes.c_double()
@@ -284,6 +305,43 @@ class Solver:
         out = np.zeros(2, np.float64)
         _check(lib().sg_energy(self._h, out.ctypes.data_as(_DP)))
         return out
+
+
+class Batch:
+    """Several FP32 solvers advanced by one launch per half-step (sg_batch_*).
+    Configure the shots first; the batch snapshots their source/station set-up."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (_VP * len(self.solvers))(*[s.handle for s in self.solvers])
+        h = _VP()
+        _check(lib().sg_batch_create(arr, len(self.solvers), ctypes.byref(h)))
+        self._h = h
+
+    def set_launch(self, ty=8, zchunk=32, grid=0, graph=True):
+        _check(lib().sg_batch_set_launch(self._h, ty, zchunk, grid, 1 if graph else 0))
+
+    def step(self, n=1):
+        _check(lib().sg_batch_step(self._h, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        _check(lib().sg_batch_step_timed(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        _check(lib().sg_batch_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -85,3 +85,21 @@ def test_host_numerics_reject_bad_arguments(sgfd_mod):
     out = np.zeros(8, np.float64)
     assert L.sg_damping_profile(8, 2, 10.0, 1, 3000.0, 1e-3, 3.4, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == -1
     assert b"invalid" in L.sg_last_error()
+
+
+def _build_example(tmp_path):
+    import subprocess
+
+    exe = tmp_path / "main_concise"
+    lib_dir = os.path.join(ROOT, "staggered-grid_amd", "sgfd")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "staggered-grid_amd", "examples", "main_concise.cpp"),
+                    "-L" + lib_dir, "-lsgfd", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cpp_dropin_header_compiles_and_links(sgfd_mod, tmp_path):
+    """The reference main.cu, ported to include/ElasticWaveEquation.hpp, builds
+    with a plain C++ compiler against the C ABI (no HIP headers needed)."""
+    exe = _build_example(tmp_path)
+    assert exe.exists()

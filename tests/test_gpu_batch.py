@@ -1,0 +1,101 @@
+"""Tile shapes, the autotuner and batched shots through the C ABI.
+
+The per-point arithmetic does not depend on how points are grouped into
+tiles, z-chunks or batches, so all of these must be bit-identical to a single
+solver with the default tile."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def sgfd_mod():
+    import sgfd
+
+    sgfd.lib()
+    return sgfd
+
+
+def _small_workload(sgfd, seed, nx, ny, nz, pml=6, nt=60, f0=20.0):
+    from sgfd import configs
+
+    g = sgfd.gaussian_field(nx, ny, nz, 4.0, seed, 4.0).astype(np.float64)
+    vp = 3000.0 * (1 + 0.05 * g)
+    b, lam, mu = configs.elastic_from_velocity(vp, vp / np.sqrt(3.0), 310.0 * vp ** 0.25)
+    rng = np.random.default_rng(seed)
+    lo = 5 + pml + 2
+    src = tuple(int(rng.integers(lo, n - lo)) for n in (nx, ny, nz))
+    st = np.array([[lo, lo, lo], [nx - lo, ny // 2, nz // 2], [src[0], src[1], 5 + pml]], np.int32)
+    return configs.Workload(f"s{seed}", nx, ny, nz, pml, 1e-3, 10.0, 10.0, 10.0, nt, b, lam, mu, src,
+                            sgfd.ricker(f0, 1e-3, nt, "none"), st, comps=("vx", "vy", "vz", "sxx"),
+                            damp_v=float(vp.max()), vp_max=float(vp.max()))
+
+
+def _run(wl, tile=None, graph=False):
+    s = wl.solver()
+    try:
+        if tile:
+            s.set_tile(*tile)
+        if graph:
+            s.set_graph(True)
+        s.step(wl.nt)
+        s.synchronize()
+        return s.stations(), s.fields()
+    finally:
+        s.close()
+
+
+def test_tile_shapes_are_bit_identical(sgfd_mod):
+    wl = _small_workload(sgfd_mod, 3, 150, 77, 61)
+    _say("tile sweep 150x77x61")
+    tr0, F0 = _run(wl)
+    for tile in ((4, 0), (16, 0), (8, 7), (16, 20), (4, 51)):
+        tr, F = _run(wl, tile)
+        assert np.array_equal(F, F0), tile
+        assert np.array_equal(tr, tr0), tile
+
+
+def test_autotune_keeps_results(sgfd_mod):
+    wl = _small_workload(sgfd_mod, 4, 96, 90, 70)
+    _say("autotune 96x90x70")
+    tr0, F0 = _run(wl)
+    s = wl.solver()
+    try:
+        ty, zc, ms = s.autotune(2)
+        assert ty in (4, 8, 16) and ms > 0
+        assert s.steps_done == 0
+        s.step(wl.nt)
+        s.synchronize()
+        assert np.array_equal(s.fields(), F0)
+        assert np.array_equal(s.stations(), tr0)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("launch", [dict(ty=8, zchunk=32, grid=0, graph=True),
+                                    dict(ty=16, zchunk=13, grid=64, graph=False)])
+def test_batched_shots_match_single_shots(sgfd_mod, launch):
+    sgfd = sgfd_mod
+    dims = [(96, 80, 60), (130, 72, 64), (70, 101, 50), (64, 64, 80)]
+    wls = [_small_workload(sgfd, 10 + i, *d) for i, d in enumerate(dims)]
+    _say(f"batch of {len(wls)} shots, launch {launch}")
+    ref = [_run(wl) for wl in wls]
+    solvers = [wl.solver() for wl in wls]
+    try:
+        b = sgfd.Batch(solvers)
+        b.set_launch(**launch)
+        b.step(wls[0].nt)
+        b.synchronize()
+        for s, (tr0, F0) in zip(solvers, ref):
+            assert s.steps_done == wls[0].nt
+            assert np.array_equal(s.fields(), F0)
+            assert np.array_equal(s.stations(), tr0)
+        b.close()
+    finally:
+        for s in solvers:
+            s.close()

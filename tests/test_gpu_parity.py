@@ -127,23 +127,78 @@ def _config1(O):
     return meta, g, p, b, lam, mu, w
 
 
+_ORACLE_CACHE = {}
+
+
+def _oracle_config1(O, p, b, lam, mu, w, nt, dtype):
+    key = (nt, np.dtype(dtype).name)
+    if key not in _ORACLE_CACHE:
+        import dataclasses
+
+        q = dataclasses.replace(p, nt=nt)
+        if dtype == np.float64:
+            _ORACLE_CACHE[key] = O.run(q, b.astype(np.float64), lam.astype(np.float64), mu.astype(np.float64),
+                                       w.astype(np.float64), None, dtype=np.float64)[1]
+        else:
+            _ORACLE_CACHE[key] = O.run(q, b, lam, mu, w, None)[1]
+    return _ORACLE_CACHE[key]
+
+
 @pytest.mark.parametrize("kernel", ["tiled", "simple"])
 def test_config1_full_run(oracle_mod, sgfd_mod, kernel):
-    """100^3 incl. a 10-cell PML, 500 steps: traces against the compiled
-    reference's golden vectors and the full final fields against the oracle."""
+    """100^3 incl. a 10-cell PML, 500 steps.
+
+    * station traces of the whole run against the compiled reference's golden
+      vectors: rel-L2 <= 1e-5;
+    * all nine wavefields at step 150 (wave inside the grid) against the
+      oracle: rel-L2 <= 1e-5;
+    * the final wavefields (step 500) are the residual left after the wave has
+      gone through the PML (~1e-3 of the peak); there the reference path's own
+      FP32 result is 3.9e-4 from the FP64 oracle, so this solver's FP32 result
+      must be at least as close to FP64 as the reference's (factor 1.5).
+    """
     O = oracle_mod
     meta, g, p, b, lam, mu, w = _config1(O)
     _say(f"config1 100^3 500 steps kernel={kernel}")
-    tr, F = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], ("vx", "vy", "vz", "sxx"), kernel=kernel,
-                   damp=_damp_for(O, p))
-    assert _rel(tr, g["traces"]) < TOL32
-    for c in range(4):
+    s = sgfd_mod.Solver(p.nx, p.ny, p.nz, p.pml, p.dt, p.dx, p.dy, p.dz, coef=p.coef, time_length=p.nt,
+                        kernel=kernel)
+    try:
+        s.set_medium(b, lam, mu)
+        s.set_damping(*_damp_for(O, p))
+        s.set_source(*p.src, w)
+        s.set_stations(g["stations"], comps=("vx", "vy", "vz", "sxx"))
+        s.step(150)
+        F150 = s.fields()
+        s.step(p.nt - 150)
+        s.synchronize()
+        tr, F = s.stations(), s.fields()
+    finally:
+        s.close()
+    # velocity traces jointly (vy is 20-40x smaller than vx/vz on this line,
+    # which lies next to the source's y-symmetry plane), then each dominant one
+    assert _rel(tr[:, :, :3], g["traces"][:, :, :3]) < TOL32
+    for c in (0, 2, 3):
         assert _rel(tr[:, :, c], g["traces"][:, :, c]) < TOL32
-    assert _rel(F[:, 2::5, 2::5, 2::5], g["fields_sub"]) < TOL32
-    _say("config1 oracle full-field run")
-    _, Fo = O.run(p, b, lam, mu, w, None, fields=True)
+    _say("config1 oracle runs (150 steps f32, 500 steps f32/f64)")
+    O150 = _oracle_config1(O, p, b, lam, mu, w, 150, np.float32)
     for f in range(9):
-        assert _rel(F[f], Fo[f]) < TOL32, sgfd_mod.FIELDS[f]
+        assert _rel(F150[f], O150[f]) < TOL32, sgfd_mod.FIELDS[f]
+    O500 = _oracle_config1(O, p, b, lam, mu, w, p.nt, np.float32)
+    O500d = _oracle_config1(O, p, b, lam, mu, w, p.nt, np.float64)
+    ref_err = _rel(O500, O500d)
+    assert ref_err < 1e-3
+    assert _rel(F, O500d) < 1.5 * ref_err
+    assert _rel(F[:, 2::5, 2::5, 2::5], g["fields_sub"]) < 2 * ref_err
+
+
+def test_config1_fp64_matches_fp64_oracle(oracle_mod, sgfd_mod):
+    O = oracle_mod
+    meta, g, p, b, lam, mu, w = _config1(O)
+    _say("config1 fp64 tiled")
+    tr, F = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], ("vx", "vy", "vz", "sxx"), precision="fp64",
+                   damp=_damp_for(O, p))
+    O500d = _oracle_config1(O, p, b, lam, mu, w, p.nt, np.float64)
+    assert _rel(F, O500d) < 1e-9
 
 
 def test_reference_cuda_station_file(sgfd_mod):
@@ -231,3 +286,19 @@ def test_invalid_inputs_raise(sgfd_mod):
         bad[100] = -1
         with pytest.raises(sgfd.SgError):
             s.set_medium(bad, bad, bad)
+
+
+def test_cpp_dropin_program_reproduces_reference_station_file(sgfd_mod, tmp_path):
+    """The reference's CUDA main program, ported to the C++ drop-in header, run
+    for 250 steps: its stationData.txt matches the reference's own file."""
+    import subprocess
+
+    from test_abi import _build_example
+
+    exe = _build_example(tmp_path)
+    _say("C++ drop-in program, 250 steps")
+    out = subprocess.run([str(exe), "250"], cwd=tmp_path, check=True, capture_output=True, text=True, timeout=120)
+    assert "recorded 250 samples" in out.stdout
+    got = np.loadtxt(tmp_path / "stationData.txt")
+    ref = np.loadtxt(os.path.join(GOLDEN, "stationData_concise_gpu.txt"))[:250]
+    assert _rel(got, ref) < 1e-4
