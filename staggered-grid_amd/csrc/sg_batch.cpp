@@ -16,6 +16,7 @@
 #include "../../include/sgfd.h"
 #include "sg_error.hpp"
 #include "sg_internal.hpp"
+#include "sg_worklist.hpp"
 
 namespace sg {
 const void *solver_shot(const sg_solver *s);
@@ -37,7 +38,7 @@ struct sg_batch {
     void *d_shots = nullptr;
     AuxDesc *d_aux = nullptr;
     WorkItem *d_items = nullptr;
-    int nitems = 0;   // padded to a multiple of 8
+    int n_in = 0, n_pml = 0;  // interior / PML items (each padded to a multiple of 8)
     int nreal = 0;
     int grid = 0;
     int ty = 8;
@@ -60,35 +61,22 @@ void drop_graph(sg_batch *b)
     b->graph_valid = false;
 }
 
-// Work-list: per shot, z-chunks of zc planes x y tiles x x tiles (x fastest);
-// then positions are interleaved so that item position p runs on XCD p % 8
-// (grid sizes are multiples of 8) and each XCD walks a contiguous band.
+// Work-lists over all shots: interior items and PML items (each interleaved
+// for the XCD round-robin), stored back to back.
 int build_items(sg_batch *b)
 {
-    std::vector<WorkItem> seq;
-    for (size_t si = 0; si < b->shots.size(); ++si) {
-        const Geom &g = solver_geom(b->shots[si]);
-        const int nbx = (g.ux + 63) / 64, nby = (g.uy + b->ty - 1) / b->ty;
-        for (int k0 = HALO; k0 < g.nz - HALO; k0 += b->zc) {
-            const int k1 = std::min(k0 + b->zc, g.nz - HALO);
-            for (int by = 0; by < nby; ++by)
-                for (int bx = 0; bx < nbx; ++bx) seq.push_back(WorkItem{(int)si, bx, by, k0, k1});
-        }
-    }
-    b->nreal = (int)seq.size();
-    const int per = (b->nreal + 7) / 8;
-    b->nitems = per * 8;
-    std::vector<WorkItem> tab(b->nitems, WorkItem{-1, 0, 0, 0, 0});
-    for (int sidx = 0; sidx < b->nreal; ++sidx) {
-        const int x = sidx / per, local = sidx % per;
-        tab[(size_t)local * 8 + x] = seq[sidx];
-    }
+    std::vector<WorkItem> in, pml;
+    for (size_t si = 0; si < b->shots.size(); ++si)
+        build_worklists(solver_geom(b->shots[si]), (int)si, b->ty, b->zc, in, pml);
+    std::vector<WorkItem> ta = xcd_order(in), tb = xcd_order(pml);
+    b->n_in = (int)ta.size();
+    b->n_pml = (int)tb.size();
+    b->nreal = (int)(in.size() + pml.size());
+    ta.insert(ta.end(), tb.begin(), tb.end());
     if (b->d_items) SG_HIP(hipFree(b->d_items));
     b->d_items = nullptr;
-    SG_HIP(hipMalloc(&b->d_items, tab.size() * sizeof(WorkItem)));
-    SG_HIP(hipMemcpy(b->d_items, tab.data(), tab.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-    if (b->grid <= 0 || b->grid > b->nitems) b->grid = b->nitems;
-    b->grid = std::max(8, b->grid / 8 * 8);
+    SG_HIP(hipMalloc(&b->d_items, ta.size() * sizeof(WorkItem)));
+    SG_HIP(hipMemcpy(b->d_items, ta.data(), ta.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     b->graph_valid = false;
     return SG_OK;
 }
@@ -96,9 +84,12 @@ int build_items(sg_batch *b)
 int enqueue(sg_batch *b)
 {
     const int n = (int)b->shots.size();
+    const int gi = b->grid > 0 ? std::min(b->grid, b->n_in) : 0, gp = b->grid > 0 ? std::min(b->grid, b->n_pml) : 0;
     SG_HIP(launch_inject_batch(b->d_shots, b->d_aux, n, b->stream));
-    SG_HIP(launch_velocity_batch(b->d_shots, b->d_items, b->nitems, b->grid, b->ty, b->stream));
-    SG_HIP(launch_stress_batch(b->d_shots, b->d_items, b->nitems, b->grid, b->ty, b->stream));
+    SG_HIP(launch_sweep(0, SG_FP32, b->ty, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
+    SG_HIP(launch_sweep(0, SG_FP32, b->ty, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
+    SG_HIP(launch_sweep(1, SG_FP32, b->ty, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
+    SG_HIP(launch_sweep(1, SG_FP32, b->ty, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
     SG_HIP(launch_gather_batch(b->d_shots, b->d_aux, n, b->max_gather, b->stream));
     return SG_OK;
 }

@@ -116,8 +116,13 @@ struct AuxDesc {
 // prec: 0 fp32, 1 fp64, 2 fp16, 3 bf16.  `shot` points to a host Shot<T,C>
 // of the matching instantiation.
 namespace sg {
-hipError_t launch_velocity(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st);
-hipError_t launch_stress(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st);
+// per-point sweeps (host Shot passed by value)
+hipError_t launch_velocity_simple(int prec, const void *shot, hipStream_t st);
+hipError_t launch_stress_simple(int prec, const void *shot, hipStream_t st);
+// tiled sweeps over a work-list: which 0 = velocity, 1 = stress; shots and
+// items in device memory; pml selects the instantiation with split updates
+hipError_t launch_sweep(int which, int prec, int ty, int pml, const void *shots_dev, const WorkItem *items,
+                        int nitems, int grid, hipStream_t st);
 hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc,
                          int namp, int accumulate, int *d_step, void *saved, hipStream_t st);
 hipError_t launch_gather(int prec, const void *shot, const long long *idx, const int *comp, int nst,
@@ -126,11 +131,6 @@ hipError_t launch_gather(int prec, const void *shot, const long long *idx, const
 hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st);
 hipError_t launch_copy(const float *src, float *dst, long long n, hipStream_t st);
 hipError_t launch_fill_zero(void *p, long long bytes, hipStream_t st);
-// batched FP32 tiled sweeps over a work-list (shots/items in device memory)
-hipError_t launch_velocity_batch(const void *shots_dev, const WorkItem *items, int nitems, int grid,
-                                 int ty, hipStream_t st);
-hipError_t launch_stress_batch(const void *shots_dev, const WorkItem *items, int nitems, int grid,
-                               int ty, hipStream_t st);
 hipError_t launch_inject_batch(const void *shots_dev, const AuxDesc *aux, int nshots, hipStream_t st);
 hipError_t launch_gather_batch(const void *shots_dev, const AuxDesc *aux, int nshots, int max_n, hipStream_t st);
 }  // namespace sg

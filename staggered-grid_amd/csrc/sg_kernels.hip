@@ -385,27 +385,25 @@ namespace sg {
     }
 
 template <class T, class C>
-static hipError_t vel_any(int kernel, const void *shot, const Tile &t, hipStream_t st)
+static hipError_t vel_any(const void *shot, hipStream_t st)
 {
-    if (kernel == 1) return vel_simple<T, C>(shot, st);
-    return velocity_tiled<T, C>(*static_cast<const Shot<T, C> *>(shot), t, st);
+    return vel_simple<T, C>(shot, st);
 }
 
 template <class T, class C>
-static hipError_t str_any(int kernel, const void *shot, const Tile &t, hipStream_t st)
+static hipError_t str_any(const void *shot, hipStream_t st)
 {
-    if (kernel == 1) return str_simple<T, C>(shot, st);
-    return stress_tiled<T, C>(*static_cast<const Shot<T, C> *>(shot), t, st);
+    return str_simple<T, C>(shot, st);
 }
 
-hipError_t launch_velocity(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st)
+hipError_t launch_velocity_simple(int prec, const void *shot, hipStream_t st)
 {
-    SG_DISPATCH(prec, vel_any, kernel, shot, t, st)
+    SG_DISPATCH(prec, vel_any, shot, st)
 }
 
-hipError_t launch_stress(int prec, int kernel, const void *shot, const Tile &t, hipStream_t st)
+hipError_t launch_stress_simple(int prec, const void *shot, hipStream_t st)
 {
-    SG_DISPATCH(prec, str_any, kernel, shot, t, st)
+    SG_DISPATCH(prec, str_any, shot, st)
 }
 
 template <class T, class C>

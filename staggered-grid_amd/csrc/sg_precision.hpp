@@ -51,4 +51,9 @@ struct Cvt<bf16, float> {
     }
 };
 
+// explicit fused multiply-add: the tiled sweeps are compiled with contraction
+// off so every instantiation (tile shape, interior/PML) rounds identically
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 }  // namespace sg

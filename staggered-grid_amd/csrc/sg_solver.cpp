@@ -22,6 +22,7 @@
 #include "sg_error.hpp"
 #include "sg_internal.hpp"
 #include "sg_precision.hpp"
+#include "sg_worklist.hpp"
 
 namespace sg {
 std::string &last_error()
@@ -123,6 +124,12 @@ struct sg_solver {
 
     std::vector<hipEvent_t> events;
 
+    // device copy of the shot descriptor and the tile work-lists (tiled sweeps)
+    void *d_shot = nullptr;
+    WorkItem *d_wl = nullptr;  // interior items, then PML items
+    int n_in = 0, n_pml = 0;
+    bool wl_valid = false;
+
     alignas(16) unsigned char shot[sizeof(Shot<double, double>)];
     float fscale[NFIELD];
     float bscale = 1, lscale = 1, mscale = 1;
@@ -170,7 +177,19 @@ void fill_shot(sg_solver *s)
     h.mscale = s->mscale;
 }
 
-void refresh_shot(sg_solver *s)
+size_t shot_size(int prec)
+{
+    switch (prec) {
+    case SG_FP32: return sizeof(Shot<float, float>);
+    case SG_FP64: return sizeof(Shot<double, double>);
+    case SG_FP16: return sizeof(Shot<_Float16, float>);
+    default: return sizeof(Shot<bf16, float>);
+    }
+}
+
+// rebuilds the host descriptor and copies it to the device (callers hold no
+// queued work on the stream: every setter synchronizes first)
+int refresh_shot(sg_solver *s)
 {
     switch (s->prec) {
     case SG_FP32: fill_shot<float, float>(s); break;
@@ -179,6 +198,28 @@ void refresh_shot(sg_solver *s)
     default: fill_shot<bf16, float>(s); break;
     }
     s->graph_valid = false;
+    if (s->d_shot) SG_HIP(hipMemcpy(s->d_shot, s->shot, shot_size(s->prec), hipMemcpyHostToDevice));
+    return SG_OK;
+}
+
+int rebuild_worklists(sg_solver *s)
+{
+    std::vector<WorkItem> in, pml;
+    const int zc = s->tile.zchunk > 0 ? s->tile.zchunk : auto_zchunk(s->g, s->tile.ty);
+    build_worklists(s->g, 0, s->tile.ty, zc, in, pml);
+    std::vector<WorkItem> a = xcd_order(in), b = xcd_order(pml);
+    s->n_in = (int)a.size();
+    s->n_pml = (int)b.size();
+    a.insert(a.end(), b.begin(), b.end());
+    if (s->d_wl) SG_HIP(hipFree(s->d_wl));
+    s->d_wl = nullptr;
+    if (!a.empty()) {
+        SG_HIP(hipMalloc(&s->d_wl, a.size() * sizeof(WorkItem)));
+        SG_HIP(hipMemcpy(s->d_wl, a.data(), a.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    }
+    s->wl_valid = true;
+    s->graph_valid = false;
+    return SG_OK;
 }
 
 void drop_graph(sg_solver *s)
@@ -219,13 +260,28 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
     const int acc = s->cfg.source_mode == SG_SOURCE_ACCUMULATE;
     SG_HIP(launch_inject(s->prec, s->shot, s->amp_d, s->amp_acc, s->namp, acc, s->d_step, s->saved, s->stream));
     if (e0) SG_HIP(hipEventRecord(e0, s->stream));
-    SG_HIP(launch_velocity(s->prec, s->cfg.kernel, s->shot, s->tile, s->stream));
-    if (e1) SG_HIP(hipEventRecord(e1, s->stream));
-    SG_HIP(launch_stress(s->prec, s->cfg.kernel, s->shot, s->tile, s->stream));
+    if (s->cfg.kernel == SG_KERNEL_SIMPLE) {
+        SG_HIP(launch_velocity_simple(s->prec, s->shot, s->stream));
+        if (e1) SG_HIP(hipEventRecord(e1, s->stream));
+        SG_HIP(launch_stress_simple(s->prec, s->shot, s->stream));
+    } else {
+        const int ty = s->tile.ty;
+        SG_HIP(launch_sweep(0, s->prec, ty, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
+        SG_HIP(launch_sweep(0, s->prec, ty, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
+        if (e1) SG_HIP(hipEventRecord(e1, s->stream));
+        SG_HIP(launch_sweep(1, s->prec, ty, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
+        SG_HIP(launch_sweep(1, s->prec, ty, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
+    }
     if (e2) SG_HIP(hipEventRecord(e2, s->stream));
     if (s->nst > 0)
         SG_HIP(launch_gather(s->prec, s->shot, s->st_idx, s->st_comp, s->nst, s->ncomp, s->interval, s->st_cap,
                              s->st_out, s->d_step, s->stream));
+    return SG_OK;
+}
+
+int ensure_worklists(sg_solver *s)
+{
+    if (s->cfg.kernel == SG_KERNEL_TILED && !s->wl_valid) return rebuild_worklists(s);
     return SG_OK;
 }
 
@@ -253,7 +309,7 @@ void free_all(sg_solver *s)
     for (auto ev : s->events) (void)hipEventDestroy(ev);
     s->events.clear();
     void *ptrs[] = {s->splits, s->tables, s->saved, s->d_step, s->amp_d, s->amp_acc,
-                    s->st_idx, s->st_comp, s->st_out, s->d_energy};
+                    s->st_idx, s->st_comp, s->st_out, s->d_energy, s->d_shot, s->d_wl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < NFIELD; ++f)
@@ -354,6 +410,7 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     SG_TRY(hipMalloc(&s->d_step, sizeof(int)));
     SG_TRY(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
     SG_TRY(hipMalloc(&s->d_energy, 2 * sizeof(double)));
+    SG_TRY(hipMalloc(&s->d_shot, sizeof(Shot<double, double>)));
     SG_TRY(hipStreamSynchronize(s->stream));
 #undef SG_TRY
 
@@ -365,7 +422,7 @@ int sg_create(const sg_config *cfg, sg_solver **out)
         if (c.pml > 0) sg_damping_profile(n3[a], c.pml, h3[a], SG_DAMP_REFERENCE, 2000.0, 1e-3, 3.4, s->damp[a].data());
     }
     if ((rc = upload_tables(s)) != SG_OK) return bail(rc);
-    refresh_shot(s);
+    if ((rc = refresh_shot(s)) != SG_OK) return bail(rc);
     *out = s;
     return SG_OK;
 }
@@ -413,7 +470,7 @@ int sg_set_medium(sg_solver *s, const float *b, const float *lam, const float *m
                 h[i] = s->prec == SG_FP16 ? to_half_bits(src[m][i] * *sc[m]) : to_bf16_bits(src[m][i] * *sc[m]);
             SG_HIP(hipMemcpy(s->model[m], h.data(), N * 2, hipMemcpyHostToDevice));
         }
-        refresh_shot(s);
+        if (int rc = refresh_shot(s)) return rc;
     }
     return SG_OK;
 }
@@ -432,8 +489,7 @@ int sg_set_field_scales(sg_solver *s, const float scales[9])
     SG_HIP(hipSetDevice(s->device));
     SG_HIP(hipStreamSynchronize(s->stream));
     for (int f = 0; f < NFIELD; ++f) s->fscale[f] = s->prec == SG_FP16 || s->prec == SG_BF16 ? scales[f] : 1.0f;
-    refresh_shot(s);
-    return SG_OK;
+    return refresh_shot(s);
 }
 
 int sg_get_field_scales(const sg_solver *s, float out[12])
@@ -498,8 +554,7 @@ int sg_set_source(sg_solver *s, int x, int y, int z, const float *wavelet, int n
         }
     }
     s->src = x + (long long)y * g.sy + (long long)z * g.sz;
-    refresh_shot(s);
-    return SG_OK;
+    return refresh_shot(s);
 }
 
 int sg_set_source_sample(sg_solver *s, int it, float value)
@@ -570,6 +625,7 @@ int sg_step(sg_solver *s, int nsteps)
     if (int rc = check_solver(s)) return rc;
     if (nsteps < 0) return fail(SG_ERR_INVALID, "sg_step: nsteps < 0");
     SG_HIP(hipSetDevice(s->device));
+    if (int rc = ensure_worklists(s)) return rc;
     if (s->use_graph) {
         if (!s->graph_valid)
             if (int rc = build_graph(s)) return rc;
@@ -700,6 +756,7 @@ int sg_step_timed(sg_solver *s, int nsteps, double *ms_vel, double *ms_str, doub
     if (int rc = check_solver(s)) return rc;
     if (nsteps < 0) return fail(SG_ERR_INVALID, "sg_step_timed: nsteps < 0");
     SG_HIP(hipSetDevice(s->device));
+    if (int rc = ensure_worklists(s)) return rc;
     const int chunk = std::min(nsteps, 2048);
     const size_t need = 3 * (size_t)chunk + 2;
     while (s->events.size() < need) {
@@ -752,6 +809,7 @@ int sg_set_tile(sg_solver *s, int tile_y, int z_chunk)
     s->tile.ty = tile_y;
     s->tile.zchunk = z_chunk;
     s->graph_valid = false;
+    s->wl_valid = false;
     return SG_OK;
 }
 
@@ -774,6 +832,7 @@ int sg_autotune(sg_solver *s, int steps, int *best_ty, int *best_zc, double *bes
             if (zc > uz) continue;
             s->tile.ty = ty;
             s->tile.zchunk = zc;
+            s->wl_valid = false;
             double v = 0, st = 0, tot = 0;
             if (int rc = sg_step_timed(s, steps, &v, &st, &tot)) {
                 s->use_graph = graph;
@@ -790,6 +849,7 @@ int sg_autotune(sg_solver *s, int steps, int *best_ty, int *best_zc, double *bes
     s->tile.ty = bty;
     s->tile.zchunk = bzc;
     s->graph_valid = false;
+    s->wl_valid = false;
     if (best_ty) *best_ty = bty;
     if (best_zc) *best_zc = bzc;
     if (best_ms) *best_ms = best / steps;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "sg_precision.hpp"
+
 namespace sg {
 
 // Split-field PML update of one component at one point.
@@ -28,6 +30,43 @@ __device__ __forceinline__ C pml_update(C fold, const C (&inc)[NA], const bool (
             so[a] = sp[a][slot];
             sn[a] = A[a] * so[a] + B[a] * inc[a];
             sp[a][slot] = sn[a];
+        } else {
+            so[a] = (C)0;
+            sn[a] = (C)0;
+        }
+    }
+    if (all) {
+        C r = sn[0];
+#pragma unroll
+        for (int a = 1; a < NA; ++a) r += sn[a];
+        return r;
+    }
+    C r = fold;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) r = dm[a] ? r - so[a] : r + inc[a];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+        if (dm[a]) r += sn[a];
+    return r;
+}
+
+// The same update with the split arrays accessed in the global address space
+// (for callers whose pointers come from a descriptor in device memory).
+template <class C, int NA>
+__device__ __forceinline__ C pml_update_g(C fold, const C (&inc)[NA], const bool (&dm)[NA], C *const (&sp)[NA],
+                                          long long slot, const C (&A)[NA], const C (&B)[NA])
+{
+    typedef __attribute__((address_space(1))) C gC;
+    C so[NA], sn[NA];
+    bool all = true;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        all = all && dm[a];
+        if (dm[a]) {
+            gC *p = (gC *)sp[a];
+            so[a] = p[slot];
+            sn[a] = fmad(A[a], so[a], B[a] * inc[a]);
+            p[slot] = sn[a];
         } else {
             so[a] = (C)0;
             sn[a] = (C)0;

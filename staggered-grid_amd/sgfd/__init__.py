@@ -17,7 +17,8 @@ from collections import namedtuple
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsgfd.so")
+# SGFD_LIB selects another build of the library (used for build-variant experiments)
+LIB_PATH = os.environ.get("SGFD_LIB", os.path.join(_HERE, "libsgfd.so"))
 
 FIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
