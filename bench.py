@@ -73,7 +73,9 @@ def main():
     ap.add_argument("--workload", default="layered512")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16", "bf16"])
     ap.add_argument("--kernel", default="tiled", choices=["tiled", "simple"])
-    ap.add_argument("--tile", type=int, nargs=2, default=None, metavar=("TY", "ZCHUNK"))
+    ap.add_argument("--tile", type=int, nargs=5, default=None, metavar=("TY", "NY", "PTY", "PNY", "ZCHUNK"),
+                    help="interior box rows and rows per thread, PML box rows and rows per thread, "
+                         "z planes per box (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
     args = ap.parse_args()

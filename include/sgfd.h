@@ -178,14 +178,25 @@ int sg_energy(sg_solver *s, double out[2]);
 int sg_step_timed(sg_solver *s, int nsteps, double *ms_velocity, double *ms_stress, double *ms_total);
 /* Captures one time step into a hipGraph and replays it (1 = on, 0 = off). */
 int sg_set_graph(sg_solver *s, int enable);
-/* Tile shape of the tiled sweeps: y rows per block (4, 8 or 16) and z planes
- * per block (>= 1; 0 = whole updated depth). */
-int sg_set_tile(sg_solver *s, int tile_y, int z_chunk);
-/* Autotuner: times `steps` steps for each tile-row count (4, 8, 16) and
- * z-chunk length (auto, 24, 48, 96, 192, whole depth), keeps the fastest
- * (per-step sweep time in *best_ms) and RESETS the wavefields (sg_reset):
- * call it before the real run.  The choice does not change any result. */
-int sg_autotune(sg_solver *s, int steps, int *best_ty, int *best_zc, double *best_ms);
+/* Box shapes of the tiled sweeps.  A box is at most 64 x tile_y points in x, y
+ * and z_chunk planes deep; a block of 64 x tile_y/rows_per_thread threads
+ * computes it.  Interior boxes (no damped point) and PML boxes have their
+ * own shape.  (tile_y, rows_per_thread) is one of (4,1) (8,1) (8,2) (16,1)
+ * (16,2); z_chunk 0 = automatic.  The shape never changes a result. */
+typedef struct sg_tile {
+    int tile_y;
+    int rows_per_thread;
+    int pml_tile_y;
+    int pml_rows_per_thread;
+    int z_chunk;
+} sg_tile;
+int sg_set_tile(sg_solver *s, const sg_tile *tile);
+int sg_get_tile(const sg_solver *s, sg_tile *tile);
+/* Autotuner: times `steps` steps for a set of interior shapes, PML shapes
+ * and z-chunk lengths, keeps the fastest (written to *best; per-step sweep
+ * time in *best_ms) and RESETS the wavefields (sg_reset): call it before the
+ * real run. */
+int sg_autotune(sg_solver *s, int steps, sg_tile *best, double *best_ms);
 /* Algorithmic HBM bytes of one velocity and one stress sweep (the lean
  * split-PML word count times the storage element size). */
 int sg_sweep_bytes(const sg_solver *s, double *velocity_bytes, double *stress_bytes);
@@ -199,10 +210,11 @@ int sg_copy_bandwidth(long long bytes, int reps, double *gbps);
  * advances all of them over a cross-shot tile work-list.  The solvers stay
  * owned by the caller and must outlive the batch. */
 int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out);
-/* Launch shape of the batched sweeps: tile rows (4, 8, 16), z planes per work
- * item, grid size (0 = one block per work item; smaller = persistent blocks
- * striding over the list) and whether one step is replayed from a hipGraph. */
-int sg_batch_set_launch(sg_batch *b, int tile_y, int z_chunk, int grid, int use_graph);
+/* Launch shape of the batched sweeps: box shapes (as sg_set_tile; z_chunk
+ * must be > 0), grid size (0 = one block per work item; smaller = persistent
+ * blocks striding over the list) and whether one step is replayed from a
+ * hipGraph. */
+int sg_batch_set_launch(sg_batch *b, const sg_tile *tile, int grid, int use_graph);
 int sg_batch_step(sg_batch *b, int nsteps);
 /* sg_batch_step bracketed by HIP events on the batch stream; waits. */
 int sg_batch_step_timed(sg_batch *b, int nsteps, double *ms);

@@ -41,8 +41,7 @@ struct sg_batch {
     int n_in = 0, n_pml = 0;  // interior / PML items (each padded to a multiple of 8)
     int nreal = 0;
     int grid = 0;
-    int ty = 8;
-    int zc = 32;
+    Tile tile{16, 1, 8, 1, 16};
     int max_gather = 0;
     bool use_graph = true;
     bool graph_valid = false;
@@ -67,7 +66,7 @@ int build_items(sg_batch *b)
 {
     std::vector<WorkItem> in, pml;
     for (size_t si = 0; si < b->shots.size(); ++si)
-        build_worklists(solver_geom(b->shots[si]), (int)si, b->ty, b->zc, in, pml);
+        build_worklists(solver_geom(b->shots[si]), (int)si, b->tile.ty, b->tile.pty, b->tile.zchunk, in, pml);
     std::vector<WorkItem> ta = xcd_order(in), tb = xcd_order(pml);
     b->n_in = (int)ta.size();
     b->n_pml = (int)tb.size();
@@ -86,10 +85,11 @@ int enqueue(sg_batch *b)
     const int n = (int)b->shots.size();
     const int gi = b->grid > 0 ? std::min(b->grid, b->n_in) : 0, gp = b->grid > 0 ? std::min(b->grid, b->n_pml) : 0;
     SG_HIP(launch_inject_batch(b->d_shots, b->d_aux, n, b->stream));
-    SG_HIP(launch_sweep(0, SG_FP32, b->ty, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
-    SG_HIP(launch_sweep(0, SG_FP32, b->ty, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
-    SG_HIP(launch_sweep(1, SG_FP32, b->ty, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
-    SG_HIP(launch_sweep(1, SG_FP32, b->ty, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
+    const Tile &t = b->tile;
+    SG_HIP(launch_sweep(0, SG_FP32, t.ty, t.ny, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
+    SG_HIP(launch_sweep(0, SG_FP32, t.pty, t.pny, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
+    SG_HIP(launch_sweep(1, SG_FP32, t.ty, t.ny, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
+    SG_HIP(launch_sweep(1, SG_FP32, t.pty, t.pny, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
     SG_HIP(launch_gather_batch(b->d_shots, b->d_aux, n, b->max_gather, b->stream));
     return SG_OK;
 }
@@ -149,15 +149,18 @@ int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out)
     return SG_OK;
 }
 
-int sg_batch_set_launch(sg_batch *b, int tile_y, int z_chunk, int grid, int use_graph)
+int sg_batch_set_launch(sg_batch *b, const sg_tile *tile, int grid, int use_graph)
 {
     if (!b) return fail(SG_ERR_INVALID, "sg_batch_set_launch: NULL batch");
-    if (tile_y != 4 && tile_y != 8 && tile_y != 16) return fail(SG_ERR_INVALID, "sg_batch_set_launch: tile_y must be 4, 8 or 16");
-    if (z_chunk < 1 || grid < 0) return fail(SG_ERR_INVALID, "sg_batch_set_launch: bad z_chunk or grid");
+    if (!tile) return fail(SG_ERR_INVALID, "sg_batch_set_launch: NULL tile");
+    if (!tile_shape_ok(tile->tile_y, tile->rows_per_thread) ||
+        !tile_shape_ok(tile->pml_tile_y, tile->pml_rows_per_thread))
+        return fail(SG_ERR_INVALID,
+                    "sg_batch_set_launch: (tile_y, rows_per_thread) must be (4,1), (8,1), (8,2), (16,1) or (16,2)");
+    if (tile->z_chunk < 1 || grid < 0) return fail(SG_ERR_INVALID, "sg_batch_set_launch: bad z_chunk or grid");
     SG_HIP(hipSetDevice(b->device));
     SG_HIP(hipStreamSynchronize(b->stream));
-    b->ty = tile_y;
-    b->zc = z_chunk;
+    b->tile = Tile{tile->tile_y, tile->rows_per_thread, tile->pml_tile_y, tile->pml_rows_per_thread, tile->z_chunk};
     b->grid = grid;
     b->use_graph = use_graph != 0;
     drop_graph(b);

@@ -83,17 +83,24 @@ struct Shot {
     float bscale, lscale, mscale;  // model storage scales (value = stored / scale)
 };
 
-// Tile description of the tiled sweeps.
+// Box shapes of the tiled sweeps (block = 64 x ty/ny threads).
 struct Tile {
-    int ty;       // y rows per block (points per thread column = ty / 4)
-    int zchunk;   // z planes per block
+    int ty, ny;    // interior boxes: y rows per box, y rows per thread
+    int pty, pny;  // PML boxes
+    int zchunk;    // z planes per box (0 = automatic)
 };
+
+inline bool tile_shape_ok(int ty, int ny)
+{
+    return (ty == 4 && ny == 1) || (ty == 8 && (ny == 1 || ny == 2)) || (ty == 16 && (ny == 1 || ny == 2));
+}
 
 // Work item of the batched launches: one (shot, x tile, y tile, z chunk).
 struct WorkItem {
     int shot;    // -1: padding entry
-    int bx, by;  // tile indices (x tiles of 64, y tiles of TY)
-    int k0, k1;  // z planes [k0, k1)
+    int i0, i1;  // x range [i0, i1), at most 64 wide
+    int j0, j1;  // y range [j0, j1), at most TY high
+    int k0, k1;  // z range [k0, k1)
 };
 
 // Per-shot state of the small kernels (source, recording) for batched steps.
@@ -121,7 +128,7 @@ hipError_t launch_velocity_simple(int prec, const void *shot, hipStream_t st);
 hipError_t launch_stress_simple(int prec, const void *shot, hipStream_t st);
 // tiled sweeps over a work-list: which 0 = velocity, 1 = stress; shots and
 // items in device memory; pml selects the instantiation with split updates
-hipError_t launch_sweep(int which, int prec, int ty, int pml, const void *shots_dev, const WorkItem *items,
+hipError_t launch_sweep(int which, int prec, int ty, int ny, int pml, const void *shots_dev, const WorkItem *items,
                         int nitems, int grid, hipStream_t st);
 hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc,
                          int namp, int accumulate, int *d_step, void *saved, hipStream_t st);

@@ -94,7 +94,7 @@ struct sg_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     Geom g{};
-    Tile tile{8, 0};
+    Tile tile{16, 1, 8, 1, 0};
 
     void *fields[NFIELD] = {};
     void *model[3] = {};
@@ -206,7 +206,7 @@ int rebuild_worklists(sg_solver *s)
 {
     std::vector<WorkItem> in, pml;
     const int zc = s->tile.zchunk > 0 ? s->tile.zchunk : auto_zchunk(s->g, s->tile.ty);
-    build_worklists(s->g, 0, s->tile.ty, zc, in, pml);
+    build_worklists(s->g, 0, s->tile.ty, s->tile.pty, zc, in, pml);
     std::vector<WorkItem> a = xcd_order(in), b = xcd_order(pml);
     s->n_in = (int)a.size();
     s->n_pml = (int)b.size();
@@ -265,12 +265,12 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
         if (e1) SG_HIP(hipEventRecord(e1, s->stream));
         SG_HIP(launch_stress_simple(s->prec, s->shot, s->stream));
     } else {
-        const int ty = s->tile.ty;
-        SG_HIP(launch_sweep(0, s->prec, ty, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
-        SG_HIP(launch_sweep(0, s->prec, ty, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
+        const Tile &t = s->tile;
+        SG_HIP(launch_sweep(0, s->prec, t.ty, t.ny, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
+        SG_HIP(launch_sweep(0, s->prec, t.pty, t.pny, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
         if (e1) SG_HIP(hipEventRecord(e1, s->stream));
-        SG_HIP(launch_sweep(1, s->prec, ty, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
-        SG_HIP(launch_sweep(1, s->prec, ty, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
+        SG_HIP(launch_sweep(1, s->prec, t.ty, t.ny, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
+        SG_HIP(launch_sweep(1, s->prec, t.pty, t.pny, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
     }
     if (e2) SG_HIP(hipEventRecord(e2, s->stream));
     if (s->nst > 0)
@@ -372,6 +372,13 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     g.zsec = 2 * P * g.uy * g.ux;
     g.ysec = g.zsec + (g.uz - 2 * P) * 2 * P * g.ux;
     g.nslots = g.ysec + (g.uz - 2 * P) * (long long)(g.uy - 2 * P) * 2 * P;
+    // the tiled sweeps address one component's splits (up to three arrays of
+    // nslots values) with 32-bit byte offsets
+    if (c.kernel == SG_KERNEL_TILED && 3 * g.nslots * (long long)compute_size(c.precision) >= (1LL << 32)) {
+        delete s;
+        return fail(SG_ERR_UNSUPPORTED, "sg_create: PML split store too large for the tiled sweeps "
+                                        "(3 * slots * element size >= 4 GiB); use SG_KERNEL_SIMPLE");
+    }
 
     auto bail = [&](int rc) {
         free_all(s);
@@ -801,57 +808,69 @@ int sg_set_graph(sg_solver *s, int enable)
     return SG_OK;
 }
 
-int sg_set_tile(sg_solver *s, int tile_y, int z_chunk)
+int sg_set_tile(sg_solver *s, const sg_tile *tile)
 {
     if (int rc = check_solver(s)) return rc;
-    if (tile_y != 4 && tile_y != 8 && tile_y != 16) return fail(SG_ERR_INVALID, "sg_set_tile: tile_y must be 4, 8 or 16");
-    if (z_chunk < 0) return fail(SG_ERR_INVALID, "sg_set_tile: z_chunk must be >= 0");
-    s->tile.ty = tile_y;
-    s->tile.zchunk = z_chunk;
+    if (!tile) return fail(SG_ERR_INVALID, "sg_set_tile: NULL tile");
+    if (!tile_shape_ok(tile->tile_y, tile->rows_per_thread) ||
+        !tile_shape_ok(tile->pml_tile_y, tile->pml_rows_per_thread))
+        return fail(SG_ERR_INVALID,
+                    "sg_set_tile: (tile_y, rows_per_thread) must be (4,1), (8,1), (8,2), (16,1) or (16,2)");
+    if (tile->z_chunk < 0) return fail(SG_ERR_INVALID, "sg_set_tile: z_chunk must be >= 0");
+    s->tile = Tile{tile->tile_y, tile->rows_per_thread, tile->pml_tile_y, tile->pml_rows_per_thread, tile->z_chunk};
     s->graph_valid = false;
     s->wl_valid = false;
     return SG_OK;
 }
 
-int sg_autotune(sg_solver *s, int steps, int *best_ty, int *best_zc, double *best_ms)
+int sg_get_tile(const sg_solver *s, sg_tile *tile)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!tile) return fail(SG_ERR_INVALID, "sg_get_tile: NULL tile");
+    *tile = sg_tile{s->tile.ty, s->tile.ny, s->tile.pty, s->tile.pny, s->tile.zchunk};
+    return SG_OK;
+}
+
+int sg_autotune(sg_solver *s, int steps, sg_tile *best_tile, double *best_ms)
 {
     if (int rc = check_solver(s)) return rc;
     if (steps < 1) return fail(SG_ERR_INVALID, "sg_autotune: steps < 1");
     if (s->cfg.kernel != SG_KERNEL_TILED) return fail(SG_ERR_STATE, "sg_autotune: the solver uses the per-point kernels");
-    // Tile rows x z-chunk length; each candidate runs `steps` timed steps
-    // from the current state, the fastest wins, then the state is reset.
+    // Two passes, each candidate timed over `steps` steps from the current
+    // state: interior shape x z-chunk length with the PML shape fixed, then
+    // the PML shape.  The state is reset afterwards.
     const int uz = s->g.uz;
-    const int zcs[] = {0, 24, 48, 96, 192, uz};
-    const int tys[] = {4, 8, 16};
+    const int shapes[][2] = {{4, 1}, {8, 1}, {16, 1}, {8, 2}, {16, 2}};
+    const int zcs[] = {0, 8, 12, 16, 24, 48};
     double best = 1e300;
-    int bty = s->tile.ty, bzc = s->tile.zchunk;
+    Tile bt = s->tile;
     const bool graph = s->use_graph;
     s->use_graph = false;
-    for (int ty : tys) {
-        for (int zc : zcs) {
-            if (zc > uz) continue;
-            s->tile.ty = ty;
-            s->tile.zchunk = zc;
-            s->wl_valid = false;
-            double v = 0, st = 0, tot = 0;
-            if (int rc = sg_step_timed(s, steps, &v, &st, &tot)) {
-                s->use_graph = graph;
-                return rc;
-            }
-            if (v + st < best) {
-                best = v + st;
-                bty = ty;
-                bzc = zc;
-            }
+    auto trial = [&](const Tile &t) -> int {
+        s->tile = t;
+        s->wl_valid = false;
+        double v = 0, st = 0, tot = 0;
+        if (int rc = sg_step_timed(s, steps, &v, &st, &tot)) return rc;
+        if (v + st < best) {
+            best = v + st;
+            bt = t;
         }
-    }
+        return SG_OK;
+    };
+    int rc = SG_OK;
+    const Tile start = s->tile;
+    for (const auto &sh : shapes)
+        for (int zc : zcs)
+            if (!rc && zc <= uz) rc = trial(Tile{sh[0], sh[1], start.pty, start.pny, zc});
+    const Tile mid = bt;
+    for (const auto &sh : shapes)
+        if (!rc && (sh[0] != mid.pty || sh[1] != mid.pny)) rc = trial(Tile{mid.ty, mid.ny, sh[0], sh[1], mid.zchunk});
     s->use_graph = graph;
-    s->tile.ty = bty;
-    s->tile.zchunk = bzc;
+    s->tile = bt;
     s->graph_valid = false;
     s->wl_valid = false;
-    if (best_ty) *best_ty = bty;
-    if (best_zc) *best_zc = bzc;
+    if (rc) return rc;
+    if (best_tile) *best_tile = sg_tile{bt.ty, bt.ny, bt.pty, bt.pny, bt.zchunk};
     if (best_ms) *best_ms = best / steps;
     return sg_reset(s);
 }

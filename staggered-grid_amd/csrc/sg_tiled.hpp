@@ -1,9 +1,9 @@
 // sg_tiled.hpp -- the tiled velocity and stress sweeps (included by sg_kernels.hip).
 //
 // Mapping to gfx950:
-//  * a block of 256 threads (64 x 4) owns an xy tile of 64 x TY points
-//    (TY = 4*NY; each thread computes NY consecutive y rows of one x column)
-//    and marches through a chunk of z planes;
+//  * a block of 64 x TY/NY threads owns an xy box of up to 64 x TY points
+//    (each thread computes NY consecutive y rows of one x column) and marches
+//    through a chunk of z planes;
 //  * the current z plane of every field that needs x or y derivatives is
 //    staged in LDS with its radius-5 halo (x rows of 74, y columns of TY+10)
 //    from x-contiguous, coalesced global reads;
@@ -16,8 +16,10 @@
 //  * loads are branch-free: out-of-grid addresses are clamped into the grid
 //    and the value masked, so many loads stay in flight; addresses are a
 //    per-plane scalar base plus a 32-bit per-thread offset;
-//  * tiles are typed: interior work items run an instantiation without any
-//    PML code, PML items one with per-point split updates;
+//  * boxes never straddle a y or z PML plane, so y/z damping is uniform per
+//    row and plane (per wave); x damping varies per lane only in the two
+//    edge tiles of a row.  Interior items run an instantiation without any
+//    PML code, PML items one with the split updates;
 //  * work items are ordered on the host so each XCD (item position % 8)
 //    walks a contiguous band of tiles, keeping halo re-reads in one L2.
 #pragma once
@@ -30,9 +32,9 @@ constexpr int TX = 64;       // x points per tile = threads in x
 constexpr int XW = TX + 10;  // LDS row width with the x halo
 constexpr int NTHR = 256;
 
-template <class C, int NY>
+template <class C, int TY>
 struct VelLds {
-    static constexpr int TY = 4 * NY, YH = TY + 10;
+    static constexpr int YH = TY + 10;
     C sxx[TY][XW];
     C sxz[TY][XW];
     C syy[YH][TX];
@@ -41,9 +43,9 @@ struct VelLds {
     C b[TY + 1][TX + 1];
 };
 
-template <class C, int NY>
+template <class C, int TY>
 struct StrLds {
-    static constexpr int TY = 4 * NY, YH = TY + 10;
+    static constexpr int YH = TY + 10;
     C vx[YH][XW];
     C vy[YH][XW];
     C vz[YH][XW];
@@ -96,18 +98,83 @@ __device__ __forceinline__ bf16 gld<bf16>(const bf16 *p, unsigned off)
     return bf16{((const __attribute__((address_space(1))) uint16_t *)p)[off]};
 }
 
+// Byte-offset access: a uniform (scalar) base plus a 32-bit per-lane byte
+// offset maps onto the global_load/store "saddr + voffset" form, with no
+// 64-bit per-lane address arithmetic and no address registers held per load.
+template <class T>
+__device__ __forceinline__ T bld(const T *p, unsigned boff)
+{
+    typedef __attribute__((address_space(1))) const char gcc;
+    typedef __attribute__((address_space(1))) const T gT;
+    return *(gT *)((gcc *)p + boff);
+}
+
+template <class T>
+__device__ __forceinline__ void bst(T *p, unsigned boff, T v)
+{
+    typedef __attribute__((address_space(1))) char gc;
+    typedef __attribute__((address_space(1))) T gT;
+    *(gT *)((gc *)p + boff) = v;
+}
+
+template <>
+__device__ __forceinline__ bf16 bld<bf16>(const bf16 *p, unsigned boff)
+{
+    typedef __attribute__((address_space(1))) const char gcc;
+    return bf16{*(__attribute__((address_space(1))) const uint16_t *)((gcc *)p + boff)};
+}
+
+template <>
+__device__ __forceinline__ void bst<bf16>(bf16 *p, unsigned boff, bf16 v)
+{
+    typedef __attribute__((address_space(1))) char gc;
+    *(__attribute__((address_space(1))) uint16_t *)((gc *)p + boff) = v.bits;
+}
+
 template <>
 __device__ __forceinline__ void gst<bf16>(bf16 *p, unsigned off, bf16 v)
 {
     ((__attribute__((address_space(1))) uint16_t *)p)[off] = v.bits;
 }
 
-// masked load: the address is always inside the array
+// masked load at a byte offset that is always inside the array
 template <class T, class C>
-__device__ __forceinline__ C mld(const T *__restrict__ p, unsigned off, bool ok, float inv)
+__device__ __forceinline__ C mld(const T *__restrict__ p, unsigned boff, bool ok, float inv)
 {
-    const C v = Cvt<T, C>::ld(gld(p, off), inv);
+    const C v = Cvt<T, C>::ld(bld(p, boff), inv);
     return ok ? v : (C)0;
+}
+
+// Split-field PML update of one component from the old splits `so` of its
+// contributing axes (loaded by the caller): damped axes f_a <- A_a f_a +
+// B_a inc_a (3DCPUwithPML.../ElasticWaveEquation3D.cpp:153-158), new splits
+// returned in `sn`.  Result: the sum of the new splits when every axis damps
+// (the reference rebuilds the component from its splits, .cpp:159), else the
+// undamped remainder (fold - old damped splits) plus undamped increments
+// plus new splits.
+template <class C, int NA>
+__device__ __forceinline__ C pml_combine(C fold, const C (&inc)[NA], const bool (&dm)[NA], const C (&so)[NA],
+                                         const C (&A)[NA], const C (&B)[NA], C (&sn)[NA])
+{
+    bool all = true;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        all = all && dm[a];
+        sn[a] = dm[a] ? fmad(A[a], so[a], B[a] * inc[a]) : (C)0;
+    }
+    if (all) {
+        C r = sn[0];
+#pragma unroll
+        for (int a = 1; a < NA; ++a) r += sn[a];
+        return r;
+    }
+    C r = fold;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) r = dm[a] ? r - so[a] : r + inc[a];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+        if (dm[a]) r += sn[a];
+    return r;
 }
 
 template <class T>
@@ -116,39 +183,43 @@ __device__ __forceinline__ T *plane(T *base, int k, long long sz)
     return base + (long long)k * sz;
 }
 
-// Per-thread geometry of a tile: row offsets inside a plane (32-bit; a plane
-// holds fewer than 2^31 points) and in/out flags.
-template <int NY>
+// Per-thread geometry of a box [i0,i1) x [j0,j1): the block is 64 x BY
+// threads (BY = TY/NY), thread (tx, ty) owns column i0+tx and rows
+// j0 + ty*NY + r.  Offsets inside a plane are 32-bit BYTE offsets (a plane
+// holds less than 4 GiB); addresses outside the array are clamped and masked.
+template <int TY, int NY>
 struct TileGeo {
+    static constexpr int BY = TY / NY;
+    static constexpr int NH = (10 + BY - 1) / BY;  // y-halo rows per thread
     int i, ic, jr[NY];
-    unsigned ro[NY];   // clamped offset of (i, jr) in a plane
-    bool rin[NY];      // (i, jr) inside the array
-    bool rup[NY];      // (i, jr) updated
-    unsigned hro[NY];  // x-halo offsets (threads tx < 10)
+    unsigned ro[NY];    // clamped offset of (i, jr) in a plane
+    bool rin[NY];       // (i, jr) inside the array
+    bool rup[NY];       // (i, jr) updated by this box
+    unsigned hro[NY];   // x-halo offsets (threads tx < 10)
     bool hok[NY];
-    int hc;            // LDS column of this thread's x-halo value
-    unsigned yro[3];   // y-halo offsets: rows h = ty, ty+4, ty+8 (h < 10)
-    bool yok[3];
-    int ylr[3];        // their LDS rows
-    unsigned bxo[NY];  // offsets of column i0+64 (thread tx == 10)
+    int hc;             // LDS column of this thread's x-halo value
+    unsigned yro[NH];   // y-halo offsets: rows h = ty + m*BY (h < 10)
+    bool yok[NH];
+    int ylr[NH];        // their LDS rows
+    unsigned bxo[NY];   // offsets of column i0+64 (thread tx == 10)
     bool bxok;
-    unsigned byo, bxyo;  // row j0+TY (ty == 0) and the corner (i0+64, j0+TY)
+    unsigned byo, bxyo; // row j0+TY (ty == 0) and the corner (i0+64, j0+TY)
     bool byok, bxyok;
 };
 
-template <int NY>
-__device__ __forceinline__ void tile_geo(const Geom &g, int i0, int j0, TileGeo<NY> &t)
+template <int TY, int NY, int ES>
+__device__ __forceinline__ void tile_geo(const Geom &g, int i0, int i1, int j0, int j1, TileGeo<TY, NY> &t)
 {
-    constexpr int TY = 4 * NY;
+    constexpr int BY = TY / NY;  // ES: bytes per stored element
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int sy = (int)g.sy;
     t.i = i0 + tx;
-    const bool xin = t.i < g.nx, xup = t.i < g.nx - HALO;
+    const bool xin = t.i < g.nx, xup = t.i >= HALO && t.i < i1;
     t.ic = xin ? t.i : g.nx - 1;
     t.hc = tx < 5 ? tx : TX + tx;
-    const int hi = i0 - HALO + t.hc;
-    const bool hx = tx < 10 && hi < g.nx;
-    const int hic = hi < g.nx ? hi : g.nx - 1;
+    const int hi = i0 - HALO + t.hc;  // negative left of the first tile
+    const bool hx = tx < 10 && hi >= 0 && hi < g.nx;
+    const int hic = hi < 0 ? 0 : (hi < g.nx ? hi : g.nx - 1);
     const int bxi = i0 + TX;
     const int bxc = bxi < g.nx ? bxi : g.nx - 1;
     t.bxok = tx == 10 && bxi < g.nx;
@@ -158,44 +229,43 @@ __device__ __forceinline__ void tile_geo(const Geom &g, int i0, int j0, TileGeo<
         t.jr[r] = j;
         const bool yin = j < g.ny;
         const int jc = yin ? j : g.ny - 1;
-        t.ro[r] = (unsigned)(t.ic + jc * sy);
+        t.ro[r] = (unsigned)(t.ic + jc * sy) * ES;
         t.rin[r] = xin && yin;
-        t.rup[r] = xup && j < g.ny - HALO;
-        t.hro[r] = (unsigned)(hic + jc * sy);
+        t.rup[r] = xup && j < j1;
+        t.hro[r] = (unsigned)(hic + jc * sy) * ES;
         t.hok[r] = hx && yin;
-        t.bxo[r] = (unsigned)(bxc + jc * sy);
+        t.bxo[r] = (unsigned)(bxc + jc * sy) * ES;
     }
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        const int h = ty + 4 * m;
+    for (int m = 0; m < TileGeo<TY, NY>::NH; ++m) {
+        const int h = ty + BY * m;
         const int lr = h < 5 ? h : TY + h;
         const int jh = j0 - HALO + lr;
         t.ylr[m] = lr;
         t.yok[m] = h < 10 && xin && jh < g.ny;
-        t.yro[m] = (unsigned)(t.ic + (jh < g.ny ? jh : g.ny - 1) * sy);
+        t.yro[m] = (unsigned)(t.ic + (jh < g.ny ? jh : g.ny - 1) * sy) * ES;
     }
     const int byj = j0 + TY;
     const int byc = byj < g.ny ? byj : g.ny - 1;
     t.byok = ty == 0 && xin && byj < g.ny;
     t.bxyok = ty == 0 && tx == 10 && bxi < g.nx && byj < g.ny;
-    t.byo = (unsigned)(t.ic + byc * sy);
-    t.bxyo = (unsigned)(bxc + byc * sy);
+    t.byo = (unsigned)(t.ic + byc * sy) * ES;
+    t.bxyo = (unsigned)(bxc + byc * sy) * ES;
 }
 
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int NY, bool PML>
-__device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int by, int k0, int k1,
-                                              VelLds<C, NY> &L)
+template <class T, class C, int TY, int NY, bool PML>
+__device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
+                                              VelLds<C, TY> &L)
 {
-    constexpr int TY = 4 * NY;
+    constexpr int BY = TY / NY, NH = TileGeo<TY, NY>::NH;
     const Geom &g = s.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = HALO + bx * TX, j0 = HALO + by * TY;
     const long long sz = g.sz;
-    TileGeo<NY> t;
-    tile_geo<NY>(g, i0, j0, t);
+    TileGeo<TY, NY> t;
+    tile_geo<TY, NY, (int)sizeof(T)>(g, i0, i1, j0, j1, t);
 
     const T *Fxx = s.f[SXX], *Fyy = s.f[SYY], *Fzz = s.f[SZZ];
     const T *Fxy = s.f[SXY], *Fxz = s.f[SXZ], *Fyz = s.f[SYZ];
@@ -225,7 +295,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
 
     // staging registers of one plane
     C c_xx[NY], c_xy[NY], c_yy[NY], h_xx[NY], h_xy[NY], h_xz[NY], b_x[NY];
-    C y_yy[3], y_yz[3], y_xy[3], b_y;
+    C y_yy[NH], y_yz[NH], y_xy[NH], b_y;
     auto fetch = [&](int k) {
         const T *pxx = plane(Fxx, k, sz), *pxy = plane(Fxy, k, sz), *pyy = plane(Fyy, k, sz);
         const T *pxz = plane(Fxz, k, sz), *pyz = plane(Fyz, k, sz), *pb = plane(Fb, k, sz);
@@ -240,7 +310,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
             b_x[r] = mld<T, C>(pb, t.bxo[r], t.bxok && t.rin[r], ib);
         }
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
+        for (int m = 0; m < NH; ++m) {
             y_yy[m] = mld<T, C>(pyy, t.yro[m], t.yok[m], fi_yy);
             y_yz[m] = mld<T, C>(pyz, t.yro[m], t.yok[m], fi_yz);
             y_xy[m] = mld<T, C>(pxy, t.yro[m], t.yok[m], fi_xy);
@@ -249,12 +319,22 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
     };
     fetch(k0);
 
-    bool px = false;
-    C A0 = (C)1, B0 = (C)1;
+    // damping: x per lane (edge tiles), y per box (y cuts at the PML planes),
+    // z per plane; the split groups of one component are `sstr` bytes apart
+    bool px = false, py = false;
+    C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
+    const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
     if (PML) {
         px = in_pml(t.i, g.nx, g.pml);
+        py = in_pml(j0, g.ny, g.pml);
         A0 = gld(s.dA[0], (unsigned)(t.ic));
         B0 = gld(s.dB[0], (unsigned)(t.ic));
+    }
+#pragma unroll
+    for (int r = 0; r < NY; ++r) {
+        const unsigned jc = (unsigned)(t.jr[r] < g.ny ? t.jr[r] : g.ny - 1);
+        A1[r] = PML ? gld(s.dA[1], jc) : (C)1;
+        B1[r] = PML ? gld(s.dB[1], jc) : (C)1;
     }
 
     for (int k = k0; k < k1; ++k) {
@@ -283,8 +363,8 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
             for (int r = 0; r < NY; ++r) L.b[ty * NY + r][TX] = b_x[r];
         }
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            if (ty + 4 * m < 10) {
+        for (int m = 0; m < NH; ++m) {
+            if (ty + BY * m < 10) {
                 L.syy[t.ylr[m]][tx] = y_yy[m];
                 L.syz[t.ylr[m]][tx] = y_yz[m];
                 L.sxy[t.ylr[m]][tx + 5] = y_xy[m];
@@ -293,7 +373,49 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
         if (ty == 0) L.b[TY][tx] = b_y;
         __syncthreads();
 
-        // ---- loads for later: plane k+1 staging, queue heads, v(k)
+        // ---- read half of the read-modify-write of plane k (issued first, so
+        // waiting for it does not wait for the prefetch below)
+        T *pv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pv[c] = plane(s.f[VX + c], k, sz);
+        C vold[3][NY];
+#pragma unroll
+        for (int r = 0; r < NY; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vold[c][r] = Cvt<T, C>::ld(bld(pv[c], t.ro[r]), s.finv[VX + c]);
+        bool pz = false, any[NY];
+        C A2 = (C)1, B2 = (C)1, so[NY][3][3];
+        unsigned sb[NY];
+        if (PML) {
+            pz = in_pml(k, g.nz, g.pml);
+            A2 = gld(s.dA[2], (unsigned)(k));
+            B2 = gld(s.dB[2], (unsigned)(k));
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                any[r] = (px || py || pz) && t.rup[r];
+                sb[r] = any[r] ? (unsigned)(split_slot(g, t.i, t.jr[r], k) * (long long)sizeof(C)) : 0u;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int a2 = 0; a2 < 3; ++a2) so[r][c][a2] = (C)0;
+                if (any[r]) {
+                    if (px) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) so[r][c][0] = bld(s.sv[3 * c], sb[r]);
+                    }
+                    if (py) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) so[r][c][1] = bld(s.sv[3 * c], sb[r] + sstr);
+                    }
+                    if (pz) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) so[r][c][2] = bld(s.sv[3 * c], sb[r] + 2 * sstr);
+                    }
+                }
+            }
+        }
+
+        // ---- loads for later: plane k+1 staging, queue heads
         if (k + 1 < k1) fetch(k + 1);
         C nxz[NY], nyz[NY], nzz[NY], nb[NY];
         {
@@ -308,23 +430,8 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
                 nb[r] = mld<T, C>(pb, t.ro[r], t.rin[r] && z2, ib);
             }
         }
-        T *pv[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) pv[c] = plane(s.f[VX + c], k, sz);
-        C vold[3][NY];
-#pragma unroll
-        for (int r = 0; r < NY; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) vold[c][r] = Cvt<T, C>::ld(gld(pv[c], t.ro[r]), s.finv[VX + c]);
 
         // ---- compute plane k
-        bool pz = false;
-        C A2 = (C)1, B2 = (C)1;
-        if (PML) {
-            pz = in_pml(k, g.nz, g.pml);
-            A2 = gld(s.dA[2], (unsigned)(k));
-            B2 = gld(s.dB[2], (unsigned)(k));
-        }
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
@@ -344,30 +451,26 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
             db[1] = s.dt * ((b0 + L.b[lr + 1][tx]) * (C)0.5);
             db[2] = s.dt * ((b0 + qb[r][1]) * (C)0.5);
             C vn[3];
-            bool done = false;
-            if (PML) {
-                const bool py = in_pml(t.jr[r], g.ny, g.pml);
-                if ((px || py || pz) && t.rup[r]) {
-                    const long long slot = split_slot(g, t.i, t.jr[r], k);
-                    const bool dm[3] = {px, py, pz};
-                    const C A[3] = {A0, gld(s.dA[1], (unsigned)t.jr[r]), A2};
-                    const C B[3] = {B0, gld(s.dB[1], (unsigned)t.jr[r]), B2};
+            if (PML && any[r]) {
+                const bool dm[3] = {px, py, pz};
+                const C A[3] = {A0, A1[r], A2};
+                const C B[3] = {B0, B1[r], B2};
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const C inc[3] = {db[c] * d[c][0], db[c] * d[c][1], db[c] * d[c][2]};
-                        C *const sp[3] = {s.sv[3 * c], s.sv[3 * c + 1], s.sv[3 * c + 2]};
-                        vn[c] = pml_update_g<C, 3>(vold[c][r], inc, dm, sp, slot, A, B);
-                    }
-                    done = true;
+                for (int c = 0; c < 3; ++c) {
+                    const C inc[3] = {db[c] * d[c][0], db[c] * d[c][1], db[c] * d[c][2]};
+                    C sn[3];
+                    vn[c] = pml_combine<C, 3>(vold[c][r], inc, dm, so[r][c], A, B, sn);
+                    if (px) bst(s.sv[3 * c], sb[r], sn[0]);
+                    if (py) bst(s.sv[3 * c], sb[r] + sstr, sn[1]);
+                    if (pz) bst(s.sv[3 * c], sb[r] + 2 * sstr, sn[2]);
                 }
-            }
-            if (!done) {
+            } else {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) vn[c] = fmad(db[c], (d[c][0] + d[c][1]) + d[c][2], vold[c][r]);
             }
             if (t.rup[r]) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) gst(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
+                for (int c = 0; c < 3; ++c) bst(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
             }
         }
         __syncthreads();
@@ -393,17 +496,16 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int bx, int b
 // ---------------------------------------------------------------------------
 // stress sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int NY, bool PML>
-__device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by, int k0, int k1,
-                                            StrLds<C, NY> &L)
+template <class T, class C, int TY, int NY, bool PML>
+__device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
+                                              StrLds<C, TY> &L)
 {
-    constexpr int TY = 4 * NY;
+    constexpr int BY = TY / NY, NH = TileGeo<TY, NY>::NH;
     const Geom &g = s.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = HALO + bx * TX, j0 = HALO + by * TY;
     const long long sz = g.sz;
-    TileGeo<NY> t;
-    tile_geo<NY>(g, i0, j0, t);
+    TileGeo<TY, NY> t;
+    tile_geo<TY, NY, (int)sizeof(T)>(g, i0, i1, j0, j1, t);
 
     const T *Fvx = s.f[VX], *Fvy = s.f[VY], *Fvz = s.f[VZ];
     const T *Fmu = s.mu, *Flam = s.lam;
@@ -462,7 +564,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
     stage_mu(k0 + 1);
 
     // staging registers: x and y halos of the three velocities
-    C h_vx[NY], h_vy[NY], h_vz[NY], y_vx[3], y_vy[3], y_vz[3];
+    C h_vx[NY], h_vy[NY], h_vz[NY], y_vx[NH], y_vy[NH], y_vz[NH];
     auto fetch = [&](int k) {
         const T *pvx = plane(Fvx, k, sz), *pvy = plane(Fvy, k, sz), *pvz = plane(Fvz, k, sz);
 #pragma unroll
@@ -472,7 +574,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
             h_vz[r] = mld<T, C>(pvz, t.hro[r], t.hok[r], fi_vz);
         }
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
+        for (int m = 0; m < NH; ++m) {
             y_vx[m] = mld<T, C>(pvx, t.yro[m], t.yok[m], fi_vx);
             y_vy[m] = mld<T, C>(pvy, t.yro[m], t.yok[m], fi_vy);
             y_vz[m] = mld<T, C>(pvz, t.yro[m], t.yok[m], fi_vz);
@@ -480,15 +582,25 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
     };
     fetch(k0);
 
-    bool px = false;
-    C A0 = (C)1, B0 = (C)1;
+    // damping as in the velocity sweep; split groups (XX, YY, ZZ: x,y,z;
+    // XY: x,y; XZ: x,z; YZ: y,z) are contiguous, `sstr` bytes apart
+    bool px = false, py = false;
+    C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
+    const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
     if (PML) {
         px = in_pml(t.i, g.nx, g.pml);
+        py = in_pml(j0, g.ny, g.pml);
         A0 = gld(s.dA[0], (unsigned)(t.ic));
         B0 = gld(s.dB[0], (unsigned)(t.ic));
     }
+#pragma unroll
+    for (int r = 0; r < NY; ++r) {
+        const unsigned jc = (unsigned)(t.jr[r] < g.ny ? t.jr[r] : g.ny - 1);
+        A1[r] = PML ? gld(s.dA[1], jc) : (C)1;
+        B1[r] = PML ? gld(s.dB[1], jc) : (C)1;
+    }
     const int src_k = s.src >= 0 ? (int)(s.src / sz) : -1;
-    const unsigned src_o = s.src >= 0 ? (unsigned)(s.src - (long long)src_k * sz) : 0u;
+    const unsigned src_o = s.src >= 0 ? (unsigned)(s.src - (long long)src_k * sz) * (unsigned)sizeof(T) : 0u;
 
     for (int k = k0; k < k1; ++k) {
 #pragma unroll
@@ -508,8 +620,8 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
             }
         }
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            if (ty + 4 * m < 10) {
+        for (int m = 0; m < NH; ++m) {
+            if (ty + BY * m < 10) {
                 L.vx[t.ylr[m]][tx + 5] = y_vx[m];
                 L.vy[t.ylr[m]][tx + 5] = y_vy[m];
                 L.vz[t.ylr[m]][tx + 5] = y_vz[m];
@@ -517,33 +629,19 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
         }
         __syncthreads();
 
-        // ---- loads for later
-        const bool more = k + 1 < k1;
-        if (more) {
-            fetch(k + 1);
-            fetch_mu(k + 2);  // staged into the ring after this plane
-        }
-        C nvx[NY], nvy[NY], nvz[NY], lam[NY];
-        {
-            const bool z6 = k + 6 < g.nz, z5 = k + 5 < g.nz;
-            const T *px_ = plane(Fvx, z6 ? k + 6 : k, sz), *py_ = plane(Fvy, z6 ? k + 6 : k, sz);
-            const T *pz_ = plane(Fvz, z5 ? k + 5 : k, sz), *pl = plane(Flam, k, sz);
-#pragma unroll
-            for (int r = 0; r < NY; ++r) {
-                nvx[r] = mld<T, C>(px_, t.ro[r], t.rin[r] && z6, fi_vx);
-                nvy[r] = mld<T, C>(py_, t.ro[r], t.rin[r] && z6, fi_vy);
-                nvz[r] = mld<T, C>(pz_, t.ro[r], t.rin[r] && z5, fi_vz);
-                lam[r] = Cvt<T, C>::ld(gld(pl, t.ro[r]), il);
-            }
-        }
+        // ---- read half of the read-modify-write of plane k, issued first
+        C lam[NY];
+        const T *pl = plane(Flam, k, sz);
         T *ps[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) ps[c] = plane(s.f[SXX + c], k, sz);
         C old[6][NY];
 #pragma unroll
-        for (int r = 0; r < NY; ++r)
+        for (int r = 0; r < NY; ++r) {
+            lam[r] = Cvt<T, C>::ld(bld(pl, t.ro[r]), il);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) old[c][r] = Cvt<T, C>::ld(gld(ps[c], t.ro[r]), s.finv[SXX + c]);
+            for (int c = 0; c < 6; ++c) old[c][r] = Cvt<T, C>::ld(bld(ps[c], t.ro[r]), s.finv[SXX + c]);
+        }
         if (k == src_k && s.src_saved) {
             // TRANSIENT source: the stress update starts from sigma before injection
 #pragma unroll
@@ -552,14 +650,68 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
 #pragma unroll
                     for (int c = 0; c < 3; ++c) old[c][r] = Cvt<T, C>::ld(gld(s.src_saved, (unsigned)c), s.finv[SXX + c]);
         }
-
-        bool pz = false;
+        bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1;
+        C sn3[NY][3][3], s2[NY][3][2];  // old splits: normal [comp][x,y,z]; shear xy, xz, yz
+        unsigned sb[NY];
         if (PML) {
             pz = in_pml(k, g.nz, g.pml);
             A2 = gld(s.dA[2], (unsigned)(k));
             B2 = gld(s.dB[2], (unsigned)(k));
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                any[r] = (px || py || pz) && t.rup[r];
+                sb[r] = any[r] ? (unsigned)(split_slot(g, t.i, t.jr[r], k) * (long long)sizeof(C)) : 0u;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                    for (int a2 = 0; a2 < 3; ++a2) sn3[r][c][a2] = (C)0;
+#pragma unroll
+                    for (int a2 = 0; a2 < 2; ++a2) s2[r][c][a2] = (C)0;
+                }
+                if (any[r]) {
+                    const unsigned o = sb[r];
+                    if (px) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) sn3[r][c][0] = bld(s.ss[XX_X + 3 * c], o);
+                        s2[r][0][0] = bld(s.ss[XY_X], o);
+                        s2[r][1][0] = bld(s.ss[XZ_X], o);
+                    }
+                    if (py) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) sn3[r][c][1] = bld(s.ss[XX_X + 3 * c], o + sstr);
+                        s2[r][0][1] = bld(s.ss[XY_X], o + sstr);
+                        s2[r][2][0] = bld(s.ss[YZ_Y], o);
+                    }
+                    if (pz) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) sn3[r][c][2] = bld(s.ss[XX_X + 3 * c], o + 2 * sstr);
+                        s2[r][1][1] = bld(s.ss[XZ_X], o + sstr);
+                        s2[r][2][1] = bld(s.ss[YZ_Y], o + sstr);
+                    }
+                }
+            }
         }
+
+        // ---- loads for later
+        const bool more = k + 1 < k1;
+        if (more) {
+            fetch(k + 1);
+            fetch_mu(k + 2);  // staged into the ring after this plane
+        }
+        C nvx[NY], nvy[NY], nvz[NY];
+        {
+            const bool z6 = k + 6 < g.nz, z5 = k + 5 < g.nz;
+            const T *px_ = plane(Fvx, z6 ? k + 6 : k, sz), *py_ = plane(Fvy, z6 ? k + 6 : k, sz);
+            const T *pz_ = plane(Fvz, z5 ? k + 5 : k, sz);
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                nvx[r] = mld<T, C>(px_, t.ro[r], t.rin[r] && z6, fi_vx);
+                nvy[r] = mld<T, C>(py_, t.ro[r], t.rin[r] && z6, fi_vy);
+                nvz[r] = mld<T, C>(pz_, t.ro[r], t.rin[r] && z5, fi_vz);
+            }
+        }
+
         const int s0 = k & 1, s1 = s0 ^ 1;
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
@@ -582,54 +734,52 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
             const C dl = s.dt * lam[r], dl2m = s.dt * (lam[r] + (C)2 * muk[r]);
             const C dmxy = s.dt * muxy, dmxz = s.dt * muxz, dmyz = s.dt * muyz;
             C nw[6];
-            bool done = false;
-            if (PML) {
-                const bool py = in_pml(t.jr[r], g.ny, g.pml);
-                if ((px || py || pz) && t.rup[r]) {
-                    const long long slot = split_slot(g, t.i, t.jr[r], k);
-                    const bool dm[3] = {px, py, pz};
-                    const C A[3] = {A0, gld(s.dA[1], (unsigned)t.jr[r]), A2};
-                    const C B[3] = {B0, gld(s.dB[1], (unsigned)t.jr[r]), B2};
-                    {
-                        const C inc[3] = {dl2m * dX, dl * dY, dl * dZ};
-                        C *const sp[3] = {s.ss[XX_X], s.ss[XX_Y], s.ss[XX_Z]};
-                        nw[0] = pml_update_g<C, 3>(old[0][r], inc, dm, sp, slot, A, B);
-                    }
-                    {
-                        const C inc[3] = {dl * dX, dl2m * dY, dl * dZ};
-                        C *const sp[3] = {s.ss[YY_X], s.ss[YY_Y], s.ss[YY_Z]};
-                        nw[1] = pml_update_g<C, 3>(old[1][r], inc, dm, sp, slot, A, B);
-                    }
-                    {
-                        const C inc[3] = {dl * dX, dl * dY, dl2m * dZ};
-                        C *const sp[3] = {s.ss[ZZ_X], s.ss[ZZ_Y], s.ss[ZZ_Z]};
-                        nw[2] = pml_update_g<C, 3>(old[2][r], inc, dm, sp, slot, A, B);
-                    }
-                    {
-                        const C inc[2] = {dmxy * dxy_x, dmxy * dxy_y};
-                        const bool m2[2] = {px, py};
-                        C *const sp[2] = {s.ss[XY_X], s.ss[XY_Y]};
-                        const C A2_[2] = {A[0], A[1]}, B2_[2] = {B[0], B[1]};
-                        nw[3] = pml_update_g<C, 2>(old[3][r], inc, m2, sp, slot, A2_, B2_);
-                    }
-                    {
-                        const C inc[2] = {dmxz * dxz_x, dmxz * dxz_z};
-                        const bool m2[2] = {px, pz};
-                        C *const sp[2] = {s.ss[XZ_X], s.ss[XZ_Z]};
-                        const C A2_[2] = {A[0], A[2]}, B2_[2] = {B[0], B[2]};
-                        nw[4] = pml_update_g<C, 2>(old[4][r], inc, m2, sp, slot, A2_, B2_);
-                    }
-                    {
-                        const C inc[2] = {dmyz * dyz_y, dmyz * dyz_z};
-                        const bool m2[2] = {py, pz};
-                        C *const sp[2] = {s.ss[YZ_Y], s.ss[YZ_Z]};
-                        const C A2_[2] = {A[1], A[2]}, B2_[2] = {B[1], B[2]};
-                        nw[5] = pml_update_g<C, 2>(old[5][r], inc, m2, sp, slot, A2_, B2_);
-                    }
-                    done = true;
+            if (PML && any[r]) {
+                const bool dm[3] = {px, py, pz};
+                const C A[3] = {A0, A1[r], A2};
+                const C B[3] = {B0, B1[r], B2};
+                const unsigned o = sb[r];
+                const C dnorm[3] = {dX, dY, dZ};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    C inc[3];
+#pragma unroll
+                    for (int a2 = 0; a2 < 3; ++a2) inc[a2] = (a2 == c ? dl2m : dl) * dnorm[a2];
+                    C sn[3];
+                    nw[c] = pml_combine<C, 3>(old[c][r], inc, dm, sn3[r][c], A, B, sn);
+                    C *const base = s.ss[XX_X + 3 * c];
+                    if (px) bst(base, o, sn[0]);
+                    if (py) bst(base, o + sstr, sn[1]);
+                    if (pz) bst(base, o + 2 * sstr, sn[2]);
                 }
-            }
-            if (!done) {
+                {
+                    const C inc[2] = {dmxy * dxy_x, dmxy * dxy_y};
+                    const bool m2[2] = {px, py};
+                    const C A2_[2] = {A[0], A[1]}, B2_[2] = {B[0], B[1]};
+                    C sn[2];
+                    nw[3] = pml_combine<C, 2>(old[3][r], inc, m2, s2[r][0], A2_, B2_, sn);
+                    if (px) bst(s.ss[XY_X], o, sn[0]);
+                    if (py) bst(s.ss[XY_X], o + sstr, sn[1]);
+                }
+                {
+                    const C inc[2] = {dmxz * dxz_x, dmxz * dxz_z};
+                    const bool m2[2] = {px, pz};
+                    const C A2_[2] = {A[0], A[2]}, B2_[2] = {B[0], B[2]};
+                    C sn[2];
+                    nw[4] = pml_combine<C, 2>(old[4][r], inc, m2, s2[r][1], A2_, B2_, sn);
+                    if (px) bst(s.ss[XZ_X], o, sn[0]);
+                    if (pz) bst(s.ss[XZ_X], o + sstr, sn[1]);
+                }
+                {
+                    const C inc[2] = {dmyz * dyz_y, dmyz * dyz_z};
+                    const bool m2[2] = {py, pz};
+                    const C A2_[2] = {A[1], A[2]}, B2_[2] = {B[1], B[2]};
+                    C sn[2];
+                    nw[5] = pml_combine<C, 2>(old[5][r], inc, m2, s2[r][2], A2_, B2_, sn);
+                    if (py) bst(s.ss[YZ_Y], o, sn[0]);
+                    if (pz) bst(s.ss[YZ_Y], o + sstr, sn[1]);
+                }
+            } else {
                 nw[0] = old[0][r] + fmad(dl2m, dX, dl * (dY + dZ));
                 nw[1] = old[1][r] + fmad(dl2m, dY, dl * (dX + dZ));
                 nw[2] = old[2][r] + fmad(dl2m, dZ, dl * (dX + dY));
@@ -639,7 +789,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
             }
             if (t.rup[r]) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) gst(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
+                for (int c = 0; c < 6; ++c) bst(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
             }
         }
         __syncthreads();
@@ -671,77 +821,69 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int bx, int by,
 // ---------------------------------------------------------------------------
 // kernels: a grid-stride loop over a work-list (one or many shots)
 // ---------------------------------------------------------------------------
-// occupancy floor per NY (waves per SIMD, caps VGPRs at 512 / waves);
-// SG_OCC_MODE 0 leaves register allocation to the compiler
-#ifndef SG_OCC_MODE
-#define SG_OCC_MODE 0
-#endif
-template <int NY>
-struct Occ {
-    static constexpr int w = SG_OCC_MODE == 0 ? 1
-                             : SG_OCC_MODE == 1 ? (NY == 1 ? 3 : 2)
-                                                : (NY == 1 ? 4 : (NY == 2 ? 3 : 2));
-};
-
-template <class T, class C, int NY, bool PML>
-__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(Occ<NY>::w))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
-                                                      const WorkItem *__restrict__ items, int nitems)
+template <class T, class C, int TY, int NY, bool PML>
+__global__ __launch_bounds__(TX *TY / NY) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
+                                                            const WorkItem *__restrict__ items, int nitems)
 {
-    __shared__ VelLds<C, NY> L;
+    __shared__ VelLds<C, TY> L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) velocity_tile<T, C, NY, PML>(shots[w.shot], w.bx, w.by, w.k0, w.k1, L);
+        if (w.shot >= 0) velocity_tile<T, C, TY, NY, PML>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
         __syncthreads();
     }
 }
 
-template <class T, class C, int NY, bool PML>
-__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(Occ<NY>::w))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
-                                                    const WorkItem *__restrict__ items, int nitems)
+template <class T, class C, int TY, int NY, bool PML>
+__global__ __launch_bounds__(TX *TY / NY) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
+                                                          const WorkItem *__restrict__ items, int nitems)
 {
-    __shared__ StrLds<C, NY> L;
+    __shared__ StrLds<C, TY> L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) stress_tile<T, C, NY, PML>(shots[w.shot], w.bx, w.by, w.k0, w.k1, L);
+        if (w.shot >= 0) stress_tile<T, C, TY, NY, PML>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
         __syncthreads();
     }
 }
 
-template <class T, class C, int NY>
-static hipError_t sweep_ny(int which, bool pml, const void *shots, const WorkItem *items, int nitems, int grid,
+template <class T, class C, int TY, int NY>
+static hipError_t sweep_tn(int which, bool pml, const void *shots, const WorkItem *items, int nitems, int grid,
                            hipStream_t st)
 {
     const Shot<T, C> *sh = static_cast<const Shot<T, C> *>(shots);
-    const dim3 block(TX, 4), g(grid);
+    const dim3 block(TX, TY / NY), g(grid);
     if (which == 0) {
-        if (pml) hipLaunchKernelGGL((k_velocity_wl<T, C, NY, true>), g, block, 0, st, sh, items, nitems);
-        else hipLaunchKernelGGL((k_velocity_wl<T, C, NY, false>), g, block, 0, st, sh, items, nitems);
+        if (pml) hipLaunchKernelGGL((k_velocity_wl<T, C, TY, NY, true>), g, block, 0, st, sh, items, nitems);
+        else hipLaunchKernelGGL((k_velocity_wl<T, C, TY, NY, false>), g, block, 0, st, sh, items, nitems);
     } else {
-        if (pml) hipLaunchKernelGGL((k_stress_wl<T, C, NY, true>), g, block, 0, st, sh, items, nitems);
-        else hipLaunchKernelGGL((k_stress_wl<T, C, NY, false>), g, block, 0, st, sh, items, nitems);
+        if (pml) hipLaunchKernelGGL((k_stress_wl<T, C, TY, NY, true>), g, block, 0, st, sh, items, nitems);
+        else hipLaunchKernelGGL((k_stress_wl<T, C, TY, NY, false>), g, block, 0, st, sh, items, nitems);
     }
     return hipGetLastError();
 }
 
+// supported (tile rows, rows per thread): (4,1) (8,1) (8,2) (16,1) (16,2)
 template <class T, class C>
-static hipError_t sweep_t(int which, int ty, bool pml, const void *shots, const WorkItem *items, int nitems,
-                          int grid, hipStream_t st)
+static hipError_t sweep_t(int which, int ty, int ny, bool pml, const void *shots, const WorkItem *items,
+                          int nitems, int grid, hipStream_t st)
 {
-    if (ty == 4) return sweep_ny<T, C, 1>(which, pml, shots, items, nitems, grid, st);
-    if (ty == 8) return sweep_ny<T, C, 2>(which, pml, shots, items, nitems, grid, st);
-    return sweep_ny<T, C, 4>(which, pml, shots, items, nitems, grid, st);
+    if (ty == 4 && ny == 1) return sweep_tn<T, C, 4, 1>(which, pml, shots, items, nitems, grid, st);
+    if (ty == 8 && ny == 1) return sweep_tn<T, C, 8, 1>(which, pml, shots, items, nitems, grid, st);
+    if (ty == 8 && ny == 2) return sweep_tn<T, C, 8, 2>(which, pml, shots, items, nitems, grid, st);
+    if (ty == 16 && ny == 2) return sweep_tn<T, C, 16, 2>(which, pml, shots, items, nitems, grid, st);
+    if (ty == 16 && ny == 1) return sweep_tn<T, C, 16, 1>(which, pml, shots, items, nitems, grid, st);
+    return hipErrorInvalidValue;
 }
 
-hipError_t launch_sweep(int which, int prec, int ty, int pml, const void *shots_dev, const WorkItem *items,
+hipError_t launch_sweep(int which, int prec, int ty, int ny, int pml, const void *shots_dev, const WorkItem *items,
                         int nitems, int grid, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
     if (grid <= 0 || grid > nitems) grid = nitems;
     switch (prec) {
-    case 0: return sweep_t<float, float>(which, ty, pml != 0, shots_dev, items, nitems, grid, st);
-    case 1: return sweep_t<double, double>(which, ty, pml != 0, shots_dev, items, nitems, grid, st);
-    case 2: return sweep_t<_Float16, float>(which, ty, pml != 0, shots_dev, items, nitems, grid, st);
-    case 3: return sweep_t<bf16, float>(which, ty, pml != 0, shots_dev, items, nitems, grid, st);
+    case 0: return sweep_t<float, float>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
+    case 1: return sweep_t<double, double>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
+    case 2: return sweep_t<_Float16, float>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
+    case 3: return sweep_t<bf16, float>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -1,55 +1,95 @@
-// sg_worklist.hpp -- host-side construction of the tile work-lists of the
-// tiled sweeps (shared by the single-shot solver and the batch).
+// sg_worklist.hpp -- host-side construction of the work-lists of the tiled
+// sweeps (shared by the single-shot solver and the batch).
 //
-// A shot's updated box is cut into 64-wide x tiles, TY-high y tiles and z
-// chunks whose boundaries include the two PML planes k = 5+P and n-5-P.  A
-// work item is INTERIOR when its x, y and z ranges lie inside the interior
-// (no PML point), else PML; the two kinds run different kernel
-// instantiations.  Items of each kind are then interleaved so that item
-// position p lands on XCD p % 8 (blocks are dispatched round-robin over the
-// 8 XCDs) while every XCD walks a contiguous band of tiles.
+// The updated range [5, n-5) of a shot is cut into boxes:
+//  * x: tiles of 64 columns at multiples of 64 (rows start on cache-line
+//    boundaries; the halo columns i < 5 are never updated); a tile holding
+//    x-PML columns damps per lane;
+//  * y and z: cut at the two PML planes (5+P and n-5-P), each of the three
+//    pieces then split into near-equal parts of at most TY rows / zc planes
+//    (z PML slabs: max(zc, min(P, 32)) planes), so damping in y and z is
+//    uniform per box and row/plane.
+// INTERIOR items (no damped column, row or plane) and PML items run
+// different kernel instantiations.  Items of each kind are interleaved so
+// that item position p lands on XCD p % 8 (blocks are dispatched round-robin
+// over the 8 XCDs) while every XCD walks a contiguous band of boxes.
 #pragma once
 
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "sg_internal.hpp"
 
 namespace sg {
 
-// z chunk length giving about `target` work items for the shot (>= 16 planes)
-inline int auto_zchunk(const Geom &g, int ty, int target = 2048)
+struct Seg {
+    int a, b;
+    bool pml;
+};
+
+// [lo, hi) in ceil(len / w) near-equal pieces
+inline void cut_even(std::vector<Seg> &out, int lo, int hi, int w, bool pml)
+{
+    if (hi <= lo) return;
+    const int len = hi - lo, n = (len + w - 1) / w;
+    for (int p = 0; p < n; ++p) out.push_back(Seg{lo + (int)((long long)len * p / n), lo + (int)((long long)len * (p + 1) / n), pml});
+}
+
+// pieces of at most w (interior) / wp (PML slabs) along one axis
+inline std::vector<Seg> axis_segments(int n, int pml, int w, int wp)
+{
+    std::vector<Seg> s;
+    const int lo = HALO, p0 = std::min(HALO + pml, n - HALO), p1 = std::max(n - HALO - pml, p0), hi = n - HALO;
+    cut_even(s, lo, p0, wp, true);
+    cut_even(s, p0, p1, w, false);
+    cut_even(s, p1, hi, wp, true);
+    return s;
+}
+
+// x tiles [64m, min(64m+64, n-5)); the kernels never update the halo
+// columns i < 5, so the first tile starts at 0 and every tile's rows start on
+// a 256-byte boundary.  pml = the tile holds a damped column.
+inline std::vector<Seg> x_tiles(int n, int pml)
+{
+    std::vector<Seg> s;
+    const int hi = n - HALO, p0 = HALO + pml, p1 = n - HALO - pml;
+    for (int a = 0; a < hi; a += 64) {
+        const int b = std::min(a + 64, hi);
+        s.push_back(Seg{a, b, std::max(a, HALO) < p0 || b > p1});
+    }
+    return s;
+}
+
+// z chunk length giving about `target` work items for the shot (>= 8 planes)
+inline int auto_zchunk(const Geom &g, int ty, int target = 8192)
 {
     const long long cols = (long long)((g.ux + 63) / 64) * ((g.uy + ty - 1) / ty);
     long long nbz = (target + cols - 1) / cols;
     if (nbz < 1) nbz = 1;
     int zc = (int)((g.uz + nbz - 1) / nbz);
-    return std::max(16, std::min(zc, g.uz));
+    return std::max(8, std::min(zc, g.uz));
 }
 
-inline void build_worklists(const Geom &g, int shot, int ty, int zc, std::vector<WorkItem> &interior,
+// Interior boxes use y pieces of at most `ty` rows, PML boxes of at most
+// `pty`; both sets of y cuts include the PML planes, so the two lists
+// partition the updated range.
+inline void build_worklists(const Geom &g, int shot, int ty, int pty, int zc, std::vector<WorkItem> &interior,
                             std::vector<WorkItem> &pml)
 {
-    const int P = g.pml;
-    const int nbx = (g.ux + 63) / 64, nby = (g.uy + ty - 1) / ty;
-    // z ranges: [5, 5+P), interior chunks, [nz-5-P, nz-5)
-    std::vector<std::pair<int, int>> zr;
-    const int zlo = HALO + P, zhi = g.nz - HALO - P;
-    for (int k = HALO; k < zlo; k += zc) zr.push_back({k, std::min(k + zc, zlo)});
-    for (int k = zlo; k < zhi; k += zc) zr.push_back({k, std::min(k + zc, zhi)});
-    for (int k = std::max(zhi, zlo); k < g.nz - HALO; k += zc) zr.push_back({k, std::min(k + zc, g.nz - HALO)});
-    for (const auto &z : zr) {
-        const bool zin = z.first >= zlo && z.second <= zhi;
-        for (int by = 0; by < nby; ++by) {
-            const int j0 = HALO + by * ty, j1 = std::min(j0 + ty, g.ny - HALO);
-            const bool yin = j0 >= HALO + P && j1 <= g.ny - HALO - P;
-            for (int bx = 0; bx < nbx; ++bx) {
-                const int i0 = HALO + bx * 64, i1 = std::min(i0 + 64, g.nx - HALO);
-                const bool xin = i0 >= HALO + P && i1 <= g.nx - HALO - P;
-                const WorkItem w{shot, bx, by, z.first, z.second};
-                (zin && yin && xin ? interior : pml).push_back(w);
-            }
-        }
+    const std::vector<Seg> xs = x_tiles(g.nx, g.pml);
+    const std::vector<Seg> ysi = axis_segments(g.ny, g.pml, ty, ty);
+    const std::vector<Seg> ysp = axis_segments(g.ny, g.pml, pty, pty);
+    // a z piece costs ten extra planes of queue warm-up: PML slabs stay whole
+    // up to 32 planes
+    const std::vector<Seg> zs = axis_segments(g.nz, g.pml, zc, std::max(zc, std::min(g.pml, 32)));
+    for (const Seg &z : zs) {
+        for (const Seg &y : ysi)
+            for (const Seg &x : xs)
+                if (!(x.pml || y.pml || z.pml)) interior.push_back(WorkItem{shot, x.a, x.b, y.a, y.b, z.a, z.b});
+        for (const Seg &y : ysp)
+            for (const Seg &x : xs)
+                if (x.pml || y.pml || z.pml) pml.push_back(WorkItem{shot, x.a, x.b, y.a, y.b, z.a, z.b});
     }
 }
 
@@ -58,7 +98,7 @@ inline std::vector<WorkItem> xcd_order(const std::vector<WorkItem> &seq)
 {
     const int n = (int)seq.size();
     const int per = (n + 7) / 8;
-    std::vector<WorkItem> tab((size_t)per * 8, WorkItem{-1, 0, 0, 0, 0});
+    std::vector<WorkItem> tab((size_t)per * 8, WorkItem{-1, 0, 0, 0, 0, 0, 0});
     for (int s = 0; s < n; ++s) tab[(size_t)(s % per) * 8 + s / per] = seq[s];
     return tab;
 }

@@ -49,6 +49,15 @@ class _Config(ctypes.Structure):
 
 _lib = None
 _FP = ctypes.POINTER(ctypes.c_float)
+class _Tile(ctypes.Structure):
+    _fields_ = [("tile_y", ctypes.c_int), ("rows_per_thread", ctypes.c_int), ("pml_tile_y", ctypes.c_int),
+                ("pml_rows_per_thread", ctypes.c_int), ("z_chunk", ctypes.c_int)]
+
+    def as_tuple(self):
+        return (self.tile_y, self.rows_per_thread, self.pml_tile_y, self.pml_rows_per_thread, self.z_chunk)
+
+
+_TP = ctypes.POINTER(_Tile)
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
 _LLP = ctypes.POINTER(ctypes.c_longlong)
@@ -78,12 +87,13 @@ API = {
     "sg_energy": (ctypes.c_int, [_VP, _DP]),
     "sg_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP]),
     "sg_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
-    "sg_set_tile": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "sg_set_tile": (ctypes.c_int, [_VP, _TP]),
+    "sg_get_tile": (ctypes.c_int, [_VP, _TP]),
     "sg_sweep_bytes": (ctypes.c_int, [_VP, _DP, _DP]),
-    "sg_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _IP, _IP, _DP]),
+    "sg_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _TP, _DP]),
     "sg_copy_bandwidth": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _DP]),
     "sg_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.POINTER(_VP)]),
-    "sg_batch_set_launch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "sg_batch_set_launch": (ctypes.c_int, [_VP, _TP, ctypes.c_int, ctypes.c_int]),
     "sg_batch_step": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_batch_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     "sg_batch_synchronize": (ctypes.c_int, [_VP]),
@@ -266,14 +276,22 @@ class Solver:
     def set_graph(self, enable=True):
         _check(lib().sg_set_graph(self._h, 1 if enable else 0))
 
-    def set_tile(self, ty, zchunk=0):
-        _check(lib().sg_set_tile(self._h, ty, zchunk))
+    def set_tile(self, ty=16, ny=1, pty=8, pny=1, zchunk=0):
+        """Box shapes: interior (ty rows per box, ny rows per thread), PML
+        (pty, pny), zchunk planes per box (0 = auto)."""
+        _check(lib().sg_set_tile(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk))))
+
+    def tile(self):
+        t = _Tile()
+        _check(lib().sg_get_tile(self._h, ctypes.byref(t)))
+        return t.as_tuple()
 
     def autotune(self, steps=5):
-        """Pick the fastest tile shape (resets the wavefields)."""
-        ty, zc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        _check(lib().sg_autotune(self._h, steps, ctypes.byref(ty), ctypes.byref(zc), ctypes.byref(ms)))
-        return ty.value, zc.value, ms.value
+        """Pick the fastest box shapes (resets the wavefields); returns
+        ((ty, ny, pty, pny, zchunk), ms per step)."""
+        t, ms = _Tile(), ctypes.c_double()
+        _check(lib().sg_autotune(self._h, steps, ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_tuple(), ms.value
 
     def sweep_bytes(self):
         v, s = ctypes.c_double(), ctypes.c_double()
@@ -319,8 +337,9 @@ class Batch:
         _check(lib().sg_batch_create(arr, len(self.solvers), ctypes.byref(h)))
         self._h = h
 
-    def set_launch(self, ty=8, zchunk=32, grid=0, graph=True):
-        _check(lib().sg_batch_set_launch(self._h, ty, zchunk, grid, 1 if graph else 0))
+    def set_launch(self, ty=16, ny=1, pty=8, pny=1, zchunk=16, grid=0, graph=True):
+        _check(lib().sg_batch_set_launch(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk)), grid,
+                                         1 if graph else 0))
 
     def step(self, n=1):
         _check(lib().sg_batch_step(self._h, int(n)))

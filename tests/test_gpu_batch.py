@@ -54,7 +54,8 @@ def test_tile_shapes_are_bit_identical(sgfd_mod):
     wl = _small_workload(sgfd_mod, 3, 150, 77, 61)
     _say("tile sweep 150x77x61")
     tr0, F0 = _run(wl)
-    for tile in ((4, 0), (16, 0), (8, 7), (16, 20), (4, 51)):
+    for tile in ((4, 1, 4, 1, 0), (8, 2, 16, 2, 0), (16, 2, 8, 1, 7), (16, 1, 16, 1, 20), (4, 1, 8, 2, 51),
+                 (8, 1, 4, 1, 13)):
         tr, F = _run(wl, tile)
         assert np.array_equal(F, F0), tile
         assert np.array_equal(tr, tr0), tile
@@ -66,8 +67,9 @@ def test_autotune_keeps_results(sgfd_mod):
     tr0, F0 = _run(wl)
     s = wl.solver()
     try:
-        ty, zc, ms = s.autotune(2)
-        assert ty in (4, 8, 16) and ms > 0
+        (ty, ny, pty, pny, zc), ms = s.autotune(2)
+        assert ty in (4, 8, 16) and pty in (4, 8, 16) and ny in (1, 2) and pny in (1, 2) and ms > 0
+        assert s.tile() == (ty, ny, pty, pny, zc)
         assert s.steps_done == 0
         s.step(wl.nt)
         s.synchronize()
@@ -77,8 +79,8 @@ def test_autotune_keeps_results(sgfd_mod):
         s.close()
 
 
-@pytest.mark.parametrize("launch", [dict(ty=8, zchunk=32, grid=0, graph=True),
-                                    dict(ty=16, zchunk=13, grid=64, graph=False)])
+@pytest.mark.parametrize("launch", [dict(ty=16, ny=1, pty=8, pny=1, zchunk=32, grid=0, graph=True),
+                                    dict(ty=16, ny=2, pty=4, pny=1, zchunk=13, grid=64, graph=False)])
 def test_batched_shots_match_single_shots(sgfd_mod, launch):
     sgfd = sgfd_mod
     dims = [(96, 80, 60), (130, 72, 64), (70, 101, 50), (64, 64, 80)]
