@@ -38,8 +38,7 @@ struct sg_batch {
     void *d_shots = nullptr;
     AuxDesc *d_aux = nullptr;
     WorkItem *d_items = nullptr;
-    int n_in = 0, n_pml = 0;  // interior / PML items (each padded to a multiple of 8)
-    int nreal = 0;
+    int off[NLIST + 1] = {0, 0, 0, 0, 0, 0};  // work-lists back to back (each padded to a multiple of 8)
     int grid = 0;
     Tile tile{16, 1, 8, 1, 16};
     int max_gather = 0;
@@ -60,22 +59,18 @@ void drop_graph(sg_batch *b)
     b->graph_valid = false;
 }
 
-// Work-lists over all shots: interior items and PML items (each interleaved
-// for the XCD round-robin), stored back to back.
+// Work-lists over all shots (one per damping mask, each interleaved for the
+// XCD round-robin), stored back to back.
 int build_items(sg_batch *b)
 {
-    std::vector<WorkItem> in, pml;
+    WorkLists wl;
     for (size_t si = 0; si < b->shots.size(); ++si)
-        build_worklists(solver_geom(b->shots[si]), (int)si, b->tile.ty, b->tile.pty, b->tile.zchunk, in, pml);
-    std::vector<WorkItem> ta = xcd_order(in), tb = xcd_order(pml);
-    b->n_in = (int)ta.size();
-    b->n_pml = (int)tb.size();
-    b->nreal = (int)(in.size() + pml.size());
-    ta.insert(ta.end(), tb.begin(), tb.end());
+        build_worklists(solver_geom(b->shots[si]), (int)si, b->tile.ty, b->tile.pty, b->tile.zchunk, wl);
+    const std::vector<WorkItem> all = pack_worklists(wl, b->off);
     if (b->d_items) SG_HIP(hipFree(b->d_items));
     b->d_items = nullptr;
-    SG_HIP(hipMalloc(&b->d_items, ta.size() * sizeof(WorkItem)));
-    SG_HIP(hipMemcpy(b->d_items, ta.data(), ta.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    SG_HIP(hipMalloc(&b->d_items, all.size() * sizeof(WorkItem)));
+    SG_HIP(hipMemcpy(b->d_items, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     b->graph_valid = false;
     return SG_OK;
 }
@@ -83,13 +78,14 @@ int build_items(sg_batch *b)
 int enqueue(sg_batch *b)
 {
     const int n = (int)b->shots.size();
-    const int gi = b->grid > 0 ? std::min(b->grid, b->n_in) : 0, gp = b->grid > 0 ? std::min(b->grid, b->n_pml) : 0;
     SG_HIP(launch_inject_batch(b->d_shots, b->d_aux, n, b->stream));
     const Tile &t = b->tile;
-    SG_HIP(launch_sweep(0, SG_FP32, t.ty, t.ny, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
-    SG_HIP(launch_sweep(0, SG_FP32, t.pty, t.pny, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
-    SG_HIP(launch_sweep(1, SG_FP32, t.ty, t.ny, 0, b->d_shots, b->d_items, b->n_in, gi, b->stream));
-    SG_HIP(launch_sweep(1, SG_FP32, t.pty, t.pny, 1, b->d_shots, b->d_items + b->n_in, b->n_pml, gp, b->stream));
+    for (int which = 0; which < 2; ++which)
+        for (int l = 0; l < NLIST; ++l) {
+            const int cnt = b->off[l + 1] - b->off[l];
+            SG_HIP(launch_sweep(which, SG_FP32, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], b->d_shots,
+                                b->d_items + b->off[l], cnt, b->grid > 0 ? std::min(b->grid, cnt) : 0, b->stream));
+        }
     SG_HIP(launch_gather_batch(b->d_shots, b->d_aux, n, b->max_gather, b->stream));
     return SG_OK;
 }

@@ -5,10 +5,14 @@
 //   * 9 wavefields and 3 model arrays, x-fastest over the total grid
 //     (index i + j*sy + k*sz, sy = nx, sz = nx*ny), like the reference CUDA
 //     INDEX macro (3DGPUwithPML_1DArraySingleGPU_concise/ElasticMacro.h:22).
-//   * PML split components only where a split exists: a compact slab store
-//     (z-slabs, then y-slabs of the z-interior, then x-slabs of the y/z
-//     interior), x-contiguous inside each slab.  Every split array spans the
-//     whole store; a split is touched only at points where its axis damps.
+//   * PML split components only where a split exists: a slab store of three
+//     sections, each starting on a 64-element boundary: the 2P z-slab planes
+//     (rows of pitch nx, indexed by absolute i and j, so split rows line up
+//     with field rows), the 2P y-slab rows of each z-interior plane (pitch
+//     nx), and the x-slabs of the y/z interior with the low and high slab
+//     stored apart (P values per row, rows of one slab contiguous).  Every
+//     split array spans the whole store; a split is touched only at points
+//     where its axis damps.
 //   * per-axis damping factor tables A(i) = (1 - dt d/2)/(1 + dt d/2) and
 //     B(i) = 1/(1 + dt d/2), formed in double on the host.
 #pragma once
@@ -35,31 +39,47 @@ struct Geom {
     int pml;             // PML thickness
     int ux, uy, uz;      // updated extents (n - 10)
     long long sy, sz;    // strides in elements
-    long long zsec;      // end of the z-slab section of the split store
-    long long ysec;      // end of the y-slab section
+    long long zsec;      // start of the y-slab section of the split store
+    long long ysec;      // start of the x-slab section
+    long long xhalf;     // size of one x slab (low, then high)
     long long nslots;    // total split slots
 };
+
+inline long long align64(long long n) { return (n + 63) / 64 * 64; }
+
+// section sizes of the split store
+inline void split_layout(Geom &g)
+{
+    const long long P = g.pml;
+    g.zsec = align64(2 * P * g.ny * (long long)g.nx);
+    g.ysec = g.zsec + align64((g.uz - 2 * P) * 2 * P * (long long)g.nx);
+    g.xhalf = align64((g.uz - 2 * P) * (g.uy - 2 * P) * P);
+    g.nslots = P > 0 ? g.ysec + 2 * g.xhalf : 0;
+}
 
 __host__ __device__ inline bool in_pml(int i, int n, int pml)
 {
     return i < HALO + pml || i >= n - HALO - pml;
 }
 
-// Compact split-store slot of an updated PML point (i, j, k) (absolute).
+// Split-store slot of an updated PML point (i, j, k) (absolute indices).
 __host__ __device__ inline long long split_slot(const Geom &g, int i, int j, int k)
 {
     const int P = g.pml;
-    const int ii = i - HALO, jj = j - HALO, kk = k - HALO;
+    const int kk = k - HALO;
     if (kk < P || kk >= g.uz - P) {
         const int kz = kk < P ? kk : kk - (g.uz - 2 * P);
-        return ((long long)kz * g.uy + jj) * g.ux + ii;
+        return ((long long)kz * g.ny + j) * g.nx + i;
     }
+    const int jj = j - HALO;
     if (jj < P || jj >= g.uy - P) {
         const int jy = jj < P ? jj : jj - (g.uy - 2 * P);
-        return g.zsec + ((long long)(kk - P) * (2 * P) + jy) * g.ux + ii;
+        return g.zsec + ((long long)(kk - P) * (2 * P) + jy) * g.nx + i;
     }
-    const int ix = ii < P ? ii : ii - (g.ux - 2 * P);
-    return g.ysec + ((long long)(kk - P) * (g.uy - 2 * P) + (jj - P)) * (2 * P) + ix;
+    const int ii = i - HALO;
+    const bool high = ii >= P;
+    const int ix = high ? ii - (g.ux - P) : ii;
+    return g.ysec + (high ? g.xhalf : 0) + ((long long)(kk - P) * (g.uy - 2 * P) + (jj - P)) * P + ix;
 }
 
 // Everything one sweep needs about one shot, passed by value to kernels (or
@@ -127,9 +147,22 @@ namespace sg {
 hipError_t launch_velocity_simple(int prec, const void *shot, hipStream_t st);
 hipError_t launch_stress_simple(int prec, const void *shot, hipStream_t st);
 // tiled sweeps over a work-list: which 0 = velocity, 1 = stress; shots and
-// items in device memory; pml selects the instantiation with split updates
-hipError_t launch_sweep(int which, int prec, int ty, int ny, int pml, const void *shots_dev, const WorkItem *items,
+// items in device memory; dm = damping mask of the list's boxes (0 interior,
+// 1/2/4 x/y/z PML face, 7 edges and corners); grid 0 = one block per item
+hipError_t launch_sweep(int which, int prec, int ty, int ny, int dm, const void *shots_dev, const WorkItem *items,
                         int nitems, int grid, hipStream_t st);
+hipError_t sweep_fp32(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                      int grid, hipStream_t st);
+hipError_t sweep_fp64(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                      int grid, hipStream_t st);
+hipError_t sweep_fp16(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                      int grid, hipStream_t st);
+hipError_t sweep_bf16(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                      int grid, hipStream_t st);
+
+// work-lists: interior, x face, y face, z face, edges/corners
+constexpr int NLIST = 5;
+constexpr int LIST_DM[NLIST] = {0, 1, 2, 4, 7};
 hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc,
                          int namp, int accumulate, int *d_step, void *saved, hipStream_t st);
 hipError_t launch_gather(int prec, const void *shot, const long long *idx, const int *comp, int nst,

@@ -335,10 +335,19 @@ __global__ __launch_bounds__(256) void k_energy(const Shot<T, C> s, double *out)
     }
 }
 
+// streaming copy, four 16-byte loads in flight per thread
 __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n4)
 {
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n4; t += (long long)gridDim.x * blockDim.x)
-        dst[t] = src[t];
+    const long long st = (long long)gridDim.x * blockDim.x;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; t + 3 * st < n4; t += 4 * st) {
+        const float4 a = src[t], b = src[t + st], c = src[t + 2 * st], d = src[t + 3 * st];
+        dst[t] = a;
+        dst[t + st] = b;
+        dst[t + 2 * st] = c;
+        dst[t + 3 * st] = d;
+    }
+    for (; t < n4; t += st) dst[t] = src[t];
 }
 
 __global__ __launch_bounds__(256) void k_zero(uint32_t *p, long long n)
@@ -369,11 +378,7 @@ static hipError_t str_simple(const void *shot, hipStream_t st)
     return hipGetLastError();
 }
 
-}  // namespace sg
 
-#include "sg_tiled.hpp"
-
-namespace sg {
 
 #define SG_DISPATCH(prec, FN, ...)                               \
     switch (prec) {                                              \
@@ -414,6 +419,18 @@ static hipError_t inject_t(const void *shot, const double *amp_d, const void *am
     hipLaunchKernelGGL((k_inject<T, C>), dim3(1), dim3(64), 0, st, s, amp_d, static_cast<const C *>(amp_acc), namp,
                        acc, d_step, static_cast<T *>(saved));
     return hipGetLastError();
+}
+
+hipError_t launch_sweep(int which, int prec, int ty, int ny, int dm, const void *shots_dev, const WorkItem *items,
+                        int nitems, int grid, hipStream_t st)
+{
+    switch (prec) {
+    case 0: return sweep_fp32(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
+    case 1: return sweep_fp64(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
+    case 2: return sweep_fp16(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
+    case 3: return sweep_bf16(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc, int namp,
@@ -471,7 +488,7 @@ hipError_t launch_gather_batch(const void *shots_dev, const AuxDesc *aux, int ns
 hipError_t launch_copy(const float *src, float *dst, long long n, hipStream_t st)
 {
     const long long n4 = n / 4;
-    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, st, reinterpret_cast<const float4 *>(src),
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, reinterpret_cast<const float4 *>(src),
                        reinterpret_cast<float4 *>(dst), n4);
     return hipGetLastError();
 }

@@ -126,8 +126,8 @@ struct sg_solver {
 
     // device copy of the shot descriptor and the tile work-lists (tiled sweeps)
     void *d_shot = nullptr;
-    WorkItem *d_wl = nullptr;  // interior items, then PML items
-    int n_in = 0, n_pml = 0;
+    WorkItem *d_wl = nullptr;  // the work-lists back to back
+    int wl_off[NLIST + 1] = {0, 0, 0, 0, 0, 0};
     bool wl_valid = false;
 
     alignas(16) unsigned char shot[sizeof(Shot<double, double>)];
@@ -204,13 +204,10 @@ int refresh_shot(sg_solver *s)
 
 int rebuild_worklists(sg_solver *s)
 {
-    std::vector<WorkItem> in, pml;
+    WorkLists wl;
     const int zc = s->tile.zchunk > 0 ? s->tile.zchunk : auto_zchunk(s->g, s->tile.ty);
-    build_worklists(s->g, 0, s->tile.ty, s->tile.pty, zc, in, pml);
-    std::vector<WorkItem> a = xcd_order(in), b = xcd_order(pml);
-    s->n_in = (int)a.size();
-    s->n_pml = (int)b.size();
-    a.insert(a.end(), b.begin(), b.end());
+    build_worklists(s->g, 0, s->tile.ty, s->tile.pty, zc, wl);
+    const std::vector<WorkItem> a = pack_worklists(wl, s->wl_off);
     if (s->d_wl) SG_HIP(hipFree(s->d_wl));
     s->d_wl = nullptr;
     if (!a.empty()) {
@@ -266,11 +263,12 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
         SG_HIP(launch_stress_simple(s->prec, s->shot, s->stream));
     } else {
         const Tile &t = s->tile;
-        SG_HIP(launch_sweep(0, s->prec, t.ty, t.ny, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
-        SG_HIP(launch_sweep(0, s->prec, t.pty, t.pny, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
-        if (e1) SG_HIP(hipEventRecord(e1, s->stream));
-        SG_HIP(launch_sweep(1, s->prec, t.ty, t.ny, 0, s->d_shot, s->d_wl, s->n_in, 0, s->stream));
-        SG_HIP(launch_sweep(1, s->prec, t.pty, t.pny, 1, s->d_shot, s->d_wl + s->n_in, s->n_pml, 0, s->stream));
+        for (int which = 0; which < 2; ++which) {
+            if (which == 1 && e1) SG_HIP(hipEventRecord(e1, s->stream));
+            for (int l = 0; l < NLIST; ++l)
+                SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], s->d_shot,
+                                    s->d_wl + s->wl_off[l], s->wl_off[l + 1] - s->wl_off[l], 0, s->stream));
+        }
     }
     if (e2) SG_HIP(hipEventRecord(e2, s->stream));
     if (s->nst > 0)
@@ -368,10 +366,7 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     g.uz = c.nz - 2 * HALO;
     g.sy = c.nx;
     g.sz = (long long)c.nx * c.ny;
-    const long long P = c.pml;
-    g.zsec = 2 * P * g.uy * g.ux;
-    g.ysec = g.zsec + (g.uz - 2 * P) * 2 * P * g.ux;
-    g.nslots = g.ysec + (g.uz - 2 * P) * (long long)(g.uy - 2 * P) * 2 * P;
+    split_layout(g);
     // the tiled sweeps address one component's splits (up to three arrays of
     // nslots values) with 32-bit byte offsets
     if (c.kernel == SG_KERNEL_TILED && 3 * g.nslots * (long long)compute_size(c.precision) >= (1LL << 32)) {

@@ -1,4 +1,5 @@
-// sg_tiled.hpp -- the tiled velocity and stress sweeps (included by sg_kernels.hip).
+// sg_tiled.hpp -- the tiled velocity and stress sweeps (instantiated once per
+// storage precision by sg_tiled_<prec>.hip).
 //
 // Mapping to gfx950:
 //  * a block of 64 x TY/NY threads owns an xy box of up to 64 x TY points
@@ -18,11 +19,19 @@
 //    per-plane scalar base plus a 32-bit per-thread offset;
 //  * boxes never straddle a y or z PML plane, so y/z damping is uniform per
 //    row and plane (per wave); x damping varies per lane only in the two
-//    edge tiles of a row.  Interior items run an instantiation without any
-//    PML code, PML items one with the split updates;
+//    edge tiles of a row.  Each box carries the mask DM of the axes that can
+//    damp inside it (1 = x, 2 = y, 4 = z): interior boxes (DM 0) run an
+//    instantiation without any PML code, the three PML faces (DM 1, 2, 4)
+//    instantiations that hold only that axis' splits, edges and corners the
+//    general one (DM 7);
 //  * work items are ordered on the host so each XCD (item position % 8)
 //    walks a contiguous band of tiles, keeping halo re-reads in one L2.
 #pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sg_internal.hpp"
+#include "sg_precision.hpp"
 
 #pragma clang fp contract(off)
 
@@ -256,7 +265,7 @@ __device__ __forceinline__ void tile_geo(const Geom &g, int i0, int i1, int j0, 
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int TY, int NY, bool PML>
+template <class T, class C, int TY, int NY, int DM>
 __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
                                               VelLds<C, TY> &L)
 {
@@ -320,21 +329,22 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     fetch(k0);
 
     // damping: x per lane (edge tiles), y per box (y cuts at the PML planes),
-    // z per plane; the split groups of one component are `sstr` bytes apart
-    bool px = false, py = false;
+    // z per plane, each only if the box's mask DM allows that axis; the
+    // split groups of one component are `sstr` bytes apart
+    constexpr bool PML = DM != 0, DX = (DM & 1) != 0, DY = (DM & 2) != 0, DZ = (DM & 4) != 0;
+    const bool px = DX && in_pml(t.i, g.nx, g.pml);
+    const bool py = DY && in_pml(j0, g.ny, g.pml);
     C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
     const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
-    if (PML) {
-        px = in_pml(t.i, g.nx, g.pml);
-        py = in_pml(j0, g.ny, g.pml);
+    if (DX) {
         A0 = gld(s.dA[0], (unsigned)(t.ic));
         B0 = gld(s.dB[0], (unsigned)(t.ic));
     }
 #pragma unroll
     for (int r = 0; r < NY; ++r) {
         const unsigned jc = (unsigned)(t.jr[r] < g.ny ? t.jr[r] : g.ny - 1);
-        A1[r] = PML ? gld(s.dA[1], jc) : (C)1;
-        B1[r] = PML ? gld(s.dB[1], jc) : (C)1;
+        A1[r] = DY ? gld(s.dA[1], jc) : (C)1;
+        B1[r] = DY ? gld(s.dB[1], jc) : (C)1;
     }
 
     for (int k = k0; k < k1; ++k) {
@@ -387,9 +397,11 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         C A2 = (C)1, B2 = (C)1, so[NY][3][3];
         unsigned sb[NY];
         if (PML) {
-            pz = in_pml(k, g.nz, g.pml);
-            A2 = gld(s.dA[2], (unsigned)(k));
-            B2 = gld(s.dB[2], (unsigned)(k));
+            if (DZ) {
+                pz = in_pml(k, g.nz, g.pml);
+                A2 = gld(s.dA[2], (unsigned)(k));
+                B2 = gld(s.dB[2], (unsigned)(k));
+            }
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 any[r] = (px || py || pz) && t.rup[r];
@@ -399,15 +411,15 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 #pragma unroll
                     for (int a2 = 0; a2 < 3; ++a2) so[r][c][a2] = (C)0;
                 if (any[r]) {
-                    if (px) {
+                    if (DX && px) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) so[r][c][0] = bld(s.sv[3 * c], sb[r]);
                     }
-                    if (py) {
+                    if (DY && py) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) so[r][c][1] = bld(s.sv[3 * c], sb[r] + sstr);
                     }
-                    if (pz) {
+                    if (DZ && pz) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) so[r][c][2] = bld(s.sv[3 * c], sb[r] + 2 * sstr);
                     }
@@ -450,27 +462,41 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
             db[0] = s.dt * ((b0 + L.b[lr][tx + 1]) * (C)0.5);
             db[1] = s.dt * ((b0 + L.b[lr + 1][tx]) * (C)0.5);
             db[2] = s.dt * ((b0 + qb[r][1]) * (C)0.5);
-            C vn[3];
-            if (PML && any[r]) {
-                const bool dm[3] = {px, py, pz};
+            C vn[3], sn[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vn[c] = fmad(db[c], (d[c][0] + d[c][1]) + d[c][2], vold[c][r]);
+            if (PML) {
+                // split update evaluated on every lane and selected where a
+                // point damps: the arithmetic is one basic block, and the
+                // split stores come after the field stores
+                const bool ar = any[r];
+                const bool dm[3] = {ar && px, ar && py, ar && pz};
                 const C A[3] = {A0, A1[r], A2};
                 const C B[3] = {B0, B1[r], B2};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const C inc[3] = {db[c] * d[c][0], db[c] * d[c][1], db[c] * d[c][2]};
-                    C sn[3];
-                    vn[c] = pml_combine<C, 3>(vold[c][r], inc, dm, so[r][c], A, B, sn);
-                    if (px) bst(s.sv[3 * c], sb[r], sn[0]);
-                    if (py) bst(s.sv[3 * c], sb[r] + sstr, sn[1]);
-                    if (pz) bst(s.sv[3 * c], sb[r] + 2 * sstr, sn[2]);
+                    const C vp = pml_combine<C, 3>(vold[c][r], inc, dm, so[r][c], A, B, sn[c]);
+                    vn[c] = ar ? vp : vn[c];
                 }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) vn[c] = fmad(db[c], (d[c][0] + d[c][1]) + d[c][2], vold[c][r]);
             }
             if (t.rup[r]) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) bst(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
+            }
+            if (PML && any[r]) {
+                if (DX && px) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst(s.sv[3 * c], sb[r], sn[c][0]);
+                }
+                if (DY && py) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst(s.sv[3 * c], sb[r] + sstr, sn[c][1]);
+                }
+                if (DZ && pz) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst(s.sv[3 * c], sb[r] + 2 * sstr, sn[c][2]);
+                }
             }
         }
         __syncthreads();
@@ -496,7 +522,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 // ---------------------------------------------------------------------------
 // stress sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int TY, int NY, bool PML>
+template <class T, class C, int TY, int NY, int DM>
 __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
                                               StrLds<C, TY> &L)
 {
@@ -584,20 +610,20 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 
     // damping as in the velocity sweep; split groups (XX, YY, ZZ: x,y,z;
     // XY: x,y; XZ: x,z; YZ: y,z) are contiguous, `sstr` bytes apart
-    bool px = false, py = false;
+    constexpr bool PML = DM != 0, DX = (DM & 1) != 0, DY = (DM & 2) != 0, DZ = (DM & 4) != 0;
+    const bool px = DX && in_pml(t.i, g.nx, g.pml);
+    const bool py = DY && in_pml(j0, g.ny, g.pml);
     C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
     const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
-    if (PML) {
-        px = in_pml(t.i, g.nx, g.pml);
-        py = in_pml(j0, g.ny, g.pml);
+    if (DX) {
         A0 = gld(s.dA[0], (unsigned)(t.ic));
         B0 = gld(s.dB[0], (unsigned)(t.ic));
     }
 #pragma unroll
     for (int r = 0; r < NY; ++r) {
         const unsigned jc = (unsigned)(t.jr[r] < g.ny ? t.jr[r] : g.ny - 1);
-        A1[r] = PML ? gld(s.dA[1], jc) : (C)1;
-        B1[r] = PML ? gld(s.dB[1], jc) : (C)1;
+        A1[r] = DY ? gld(s.dA[1], jc) : (C)1;
+        B1[r] = DY ? gld(s.dB[1], jc) : (C)1;
     }
     const int src_k = s.src >= 0 ? (int)(s.src / sz) : -1;
     const unsigned src_o = s.src >= 0 ? (unsigned)(s.src - (long long)src_k * sz) * (unsigned)sizeof(T) : 0u;
@@ -655,9 +681,11 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         C sn3[NY][3][3], s2[NY][3][2];  // old splits: normal [comp][x,y,z]; shear xy, xz, yz
         unsigned sb[NY];
         if (PML) {
-            pz = in_pml(k, g.nz, g.pml);
-            A2 = gld(s.dA[2], (unsigned)(k));
-            B2 = gld(s.dB[2], (unsigned)(k));
+            if (DZ) {
+                pz = in_pml(k, g.nz, g.pml);
+                A2 = gld(s.dA[2], (unsigned)(k));
+                B2 = gld(s.dB[2], (unsigned)(k));
+            }
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 any[r] = (px || py || pz) && t.rup[r];
@@ -671,19 +699,19 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 }
                 if (any[r]) {
                     const unsigned o = sb[r];
-                    if (px) {
+                    if (DX && px) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) sn3[r][c][0] = bld(s.ss[XX_X + 3 * c], o);
                         s2[r][0][0] = bld(s.ss[XY_X], o);
                         s2[r][1][0] = bld(s.ss[XZ_X], o);
                     }
-                    if (py) {
+                    if (DY && py) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) sn3[r][c][1] = bld(s.ss[XX_X + 3 * c], o + sstr);
                         s2[r][0][1] = bld(s.ss[XY_X], o + sstr);
                         s2[r][2][0] = bld(s.ss[YZ_Y], o);
                     }
-                    if (pz) {
+                    if (DZ && pz) {
 #pragma unroll
                         for (int c = 0; c < 3; ++c) sn3[r][c][2] = bld(s.ss[XX_X + 3 * c], o + 2 * sstr);
                         s2[r][1][1] = bld(s.ss[XZ_X], o + sstr);
@@ -734,62 +762,76 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
             const C dl = s.dt * lam[r], dl2m = s.dt * (lam[r] + (C)2 * muk[r]);
             const C dmxy = s.dt * muxy, dmxz = s.dt * muxz, dmyz = s.dt * muyz;
             C nw[6];
-            if (PML && any[r]) {
-                const bool dm[3] = {px, py, pz};
+            nw[0] = old[0][r] + fmad(dl2m, dX, dl * (dY + dZ));
+            nw[1] = old[1][r] + fmad(dl2m, dY, dl * (dX + dZ));
+            nw[2] = old[2][r] + fmad(dl2m, dZ, dl * (dX + dY));
+            nw[3] = fmad(dmxy, dxy_x + dxy_y, old[3][r]);
+            nw[4] = fmad(dmxz, dxz_x + dxz_z, old[4][r]);
+            nw[5] = fmad(dmyz, dyz_y + dyz_z, old[5][r]);
+            C sn[3][3], t2[3][2];  // new splits: normal [comp][x,y,z]; shear xy, xz, yz
+            if (PML) {
+                // split update on every lane, selected where a point damps;
+                // split stores follow the field stores
+                const bool ar = any[r];
+                const bool mx = ar && px, my = ar && py, mz = ar && pz;
+                const bool dm[3] = {mx, my, mz};
                 const C A[3] = {A0, A1[r], A2};
                 const C B[3] = {B0, B1[r], B2};
-                const unsigned o = sb[r];
                 const C dnorm[3] = {dX, dY, dZ};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     C inc[3];
 #pragma unroll
                     for (int a2 = 0; a2 < 3; ++a2) inc[a2] = (a2 == c ? dl2m : dl) * dnorm[a2];
-                    C sn[3];
-                    nw[c] = pml_combine<C, 3>(old[c][r], inc, dm, sn3[r][c], A, B, sn);
-                    C *const base = s.ss[XX_X + 3 * c];
-                    if (px) bst(base, o, sn[0]);
-                    if (py) bst(base, o + sstr, sn[1]);
-                    if (pz) bst(base, o + 2 * sstr, sn[2]);
+                    const C vp = pml_combine<C, 3>(old[c][r], inc, dm, sn3[r][c], A, B, sn[c]);
+                    nw[c] = ar ? vp : nw[c];
                 }
                 {
                     const C inc[2] = {dmxy * dxy_x, dmxy * dxy_y};
-                    const bool m2[2] = {px, py};
+                    const bool m2[2] = {mx, my};
                     const C A2_[2] = {A[0], A[1]}, B2_[2] = {B[0], B[1]};
-                    C sn[2];
-                    nw[3] = pml_combine<C, 2>(old[3][r], inc, m2, s2[r][0], A2_, B2_, sn);
-                    if (px) bst(s.ss[XY_X], o, sn[0]);
-                    if (py) bst(s.ss[XY_X], o + sstr, sn[1]);
+                    const C vp = pml_combine<C, 2>(old[3][r], inc, m2, s2[r][0], A2_, B2_, t2[0]);
+                    nw[3] = ar ? vp : nw[3];
                 }
                 {
                     const C inc[2] = {dmxz * dxz_x, dmxz * dxz_z};
-                    const bool m2[2] = {px, pz};
+                    const bool m2[2] = {mx, mz};
                     const C A2_[2] = {A[0], A[2]}, B2_[2] = {B[0], B[2]};
-                    C sn[2];
-                    nw[4] = pml_combine<C, 2>(old[4][r], inc, m2, s2[r][1], A2_, B2_, sn);
-                    if (px) bst(s.ss[XZ_X], o, sn[0]);
-                    if (pz) bst(s.ss[XZ_X], o + sstr, sn[1]);
+                    const C vp = pml_combine<C, 2>(old[4][r], inc, m2, s2[r][1], A2_, B2_, t2[1]);
+                    nw[4] = ar ? vp : nw[4];
                 }
                 {
                     const C inc[2] = {dmyz * dyz_y, dmyz * dyz_z};
-                    const bool m2[2] = {py, pz};
+                    const bool m2[2] = {my, mz};
                     const C A2_[2] = {A[1], A[2]}, B2_[2] = {B[1], B[2]};
-                    C sn[2];
-                    nw[5] = pml_combine<C, 2>(old[5][r], inc, m2, s2[r][2], A2_, B2_, sn);
-                    if (py) bst(s.ss[YZ_Y], o, sn[0]);
-                    if (pz) bst(s.ss[YZ_Y], o + sstr, sn[1]);
+                    const C vp = pml_combine<C, 2>(old[5][r], inc, m2, s2[r][2], A2_, B2_, t2[2]);
+                    nw[5] = ar ? vp : nw[5];
                 }
-            } else {
-                nw[0] = old[0][r] + fmad(dl2m, dX, dl * (dY + dZ));
-                nw[1] = old[1][r] + fmad(dl2m, dY, dl * (dX + dZ));
-                nw[2] = old[2][r] + fmad(dl2m, dZ, dl * (dX + dY));
-                nw[3] = fmad(dmxy, dxy_x + dxy_y, old[3][r]);
-                nw[4] = fmad(dmxz, dxz_x + dxz_z, old[4][r]);
-                nw[5] = fmad(dmyz, dyz_y + dyz_z, old[5][r]);
             }
             if (t.rup[r]) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) bst(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
+            }
+            if (PML && any[r]) {
+                const unsigned o = sb[r];
+                if (DX && px) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst(s.ss[XX_X + 3 * c], o, sn[c][0]);
+                    bst(s.ss[XY_X], o, t2[0][0]);
+                    bst(s.ss[XZ_X], o, t2[1][0]);
+                }
+                if (DY && py) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst(s.ss[XX_X + 3 * c], o + sstr, sn[c][1]);
+                    bst(s.ss[XY_X], o + sstr, t2[0][1]);
+                    bst(s.ss[YZ_Y], o, t2[2][0]);
+                }
+                if (DZ && pz) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst(s.ss[XX_X + 3 * c], o + 2 * sstr, sn[c][2]);
+                    bst(s.ss[XZ_X], o + sstr, t2[1][1]);
+                    bst(s.ss[YZ_Y], o + sstr, t2[2][1]);
+                }
             }
         }
         __syncthreads();
@@ -821,71 +863,78 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 // ---------------------------------------------------------------------------
 // kernels: a grid-stride loop over a work-list (one or many shots)
 // ---------------------------------------------------------------------------
-template <class T, class C, int TY, int NY, bool PML>
-__global__ __launch_bounds__(TX *TY / NY) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
+// PML instantiations may be held to SG_PML_WAVES waves per SIMD (register cap)
+#ifndef SG_PML_WAVES
+#define SG_PML_WAVES 1
+#endif
+template <int DM>
+struct WavesCap {
+    static constexpr int v = DM ? SG_PML_WAVES : 1;
+};
+
+template <class T, class C, int TY, int NY, int DM>
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM>::v))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
                                                             const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ VelLds<C, TY> L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) velocity_tile<T, C, TY, NY, PML>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        if (w.shot >= 0) velocity_tile<T, C, TY, NY, DM>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
         __syncthreads();
     }
 }
 
-template <class T, class C, int TY, int NY, bool PML>
-__global__ __launch_bounds__(TX *TY / NY) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
+template <class T, class C, int TY, int NY, int DM>
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM>::v))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
                                                           const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ StrLds<C, TY> L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) stress_tile<T, C, TY, NY, PML>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        if (w.shot >= 0) stress_tile<T, C, TY, NY, DM>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
         __syncthreads();
     }
 }
 
-template <class T, class C, int TY, int NY>
-static hipError_t sweep_tn(int which, bool pml, const void *shots, const WorkItem *items, int nitems, int grid,
-                           hipStream_t st)
+template <class T, class C, int TY, int NY, int DM>
+static hipError_t sweep_tnd(int which, const void *shots, const WorkItem *items, int nitems, int grid,
+                            hipStream_t st)
 {
     const Shot<T, C> *sh = static_cast<const Shot<T, C> *>(shots);
     const dim3 block(TX, TY / NY), g(grid);
-    if (which == 0) {
-        if (pml) hipLaunchKernelGGL((k_velocity_wl<T, C, TY, NY, true>), g, block, 0, st, sh, items, nitems);
-        else hipLaunchKernelGGL((k_velocity_wl<T, C, TY, NY, false>), g, block, 0, st, sh, items, nitems);
-    } else {
-        if (pml) hipLaunchKernelGGL((k_stress_wl<T, C, TY, NY, true>), g, block, 0, st, sh, items, nitems);
-        else hipLaunchKernelGGL((k_stress_wl<T, C, TY, NY, false>), g, block, 0, st, sh, items, nitems);
-    }
+    if (which == 0) hipLaunchKernelGGL((k_velocity_wl<T, C, TY, NY, DM>), g, block, 0, st, sh, items, nitems);
+    else hipLaunchKernelGGL((k_stress_wl<T, C, TY, NY, DM>), g, block, 0, st, sh, items, nitems);
     return hipGetLastError();
 }
 
-// supported (tile rows, rows per thread): (4,1) (8,1) (8,2) (16,1) (16,2)
-template <class T, class C>
-static hipError_t sweep_t(int which, int ty, int ny, bool pml, const void *shots, const WorkItem *items,
-                          int nitems, int grid, hipStream_t st)
+template <class T, class C, int TY, int NY>
+static hipError_t sweep_tn(int which, int dm, const void *shots, const WorkItem *items, int nitems, int grid,
+                           hipStream_t st)
 {
-    if (ty == 4 && ny == 1) return sweep_tn<T, C, 4, 1>(which, pml, shots, items, nitems, grid, st);
-    if (ty == 8 && ny == 1) return sweep_tn<T, C, 8, 1>(which, pml, shots, items, nitems, grid, st);
-    if (ty == 8 && ny == 2) return sweep_tn<T, C, 8, 2>(which, pml, shots, items, nitems, grid, st);
-    if (ty == 16 && ny == 2) return sweep_tn<T, C, 16, 2>(which, pml, shots, items, nitems, grid, st);
-    if (ty == 16 && ny == 1) return sweep_tn<T, C, 16, 1>(which, pml, shots, items, nitems, grid, st);
-    return hipErrorInvalidValue;
+    switch (dm) {
+    case 0: return sweep_tnd<T, C, TY, NY, 0>(which, shots, items, nitems, grid, st);
+    case 1: return sweep_tnd<T, C, TY, NY, 1>(which, shots, items, nitems, grid, st);
+    case 2: return sweep_tnd<T, C, TY, NY, 2>(which, shots, items, nitems, grid, st);
+    case 4: return sweep_tnd<T, C, TY, NY, 4>(which, shots, items, nitems, grid, st);
+    case 7: return sweep_tnd<T, C, TY, NY, 7>(which, shots, items, nitems, grid, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
-hipError_t launch_sweep(int which, int prec, int ty, int ny, int pml, const void *shots_dev, const WorkItem *items,
-                        int nitems, int grid, hipStream_t st)
+// supported (tile rows, rows per thread): (4,1) (8,1) (8,2) (16,1) (16,2);
+// damping masks 0, 1, 2, 4, 7
+template <class T, class C>
+hipError_t sweep_t(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                   int grid, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
     if (grid <= 0 || grid > nitems) grid = nitems;
-    switch (prec) {
-    case 0: return sweep_t<float, float>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
-    case 1: return sweep_t<double, double>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
-    case 2: return sweep_t<_Float16, float>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
-    case 3: return sweep_t<bf16, float>(which, ty, ny, pml != 0, shots_dev, items, nitems, grid, st);
-    default: return hipErrorInvalidValue;
-    }
+    if (ty == 4 && ny == 1) return sweep_tn<T, C, 4, 1>(which, dm, shots, items, nitems, grid, st);
+    if (ty == 8 && ny == 1) return sweep_tn<T, C, 8, 1>(which, dm, shots, items, nitems, grid, st);
+    if (ty == 8 && ny == 2) return sweep_tn<T, C, 8, 2>(which, dm, shots, items, nitems, grid, st);
+    if (ty == 16 && ny == 1) return sweep_tn<T, C, 16, 1>(which, dm, shots, items, nitems, grid, st);
+    if (ty == 16 && ny == 2) return sweep_tn<T, C, 16, 2>(which, dm, shots, items, nitems, grid, st);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace sg

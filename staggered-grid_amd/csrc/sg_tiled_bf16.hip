@@ -1,0 +1,12 @@
+// sg_tiled_bf16.hip -- tiled sweeps with bf16 storage and float arithmetic.
+#include "sg_tiled.hpp"
+
+namespace sg {
+
+hipError_t sweep_bf16(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                     int grid, hipStream_t st)
+{
+    return sweep_t<bf16, float>(which, ty, ny, dm, shots, items, nitems, grid, st);
+}
+
+}  // namespace sg

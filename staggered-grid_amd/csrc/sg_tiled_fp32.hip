@@ -1,0 +1,12 @@
+// sg_tiled_fp32.hip -- tiled sweeps with float storage and float arithmetic.
+#include "sg_tiled.hpp"
+
+namespace sg {
+
+hipError_t sweep_fp32(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                     int grid, hipStream_t st)
+{
+    return sweep_t<float, float>(which, ty, ny, dm, shots, items, nitems, grid, st);
+}
+
+}  // namespace sg

@@ -1,0 +1,12 @@
+// sg_tiled_fp64.hip -- tiled sweeps with double storage and double arithmetic.
+#include "sg_tiled.hpp"
+
+namespace sg {
+
+hipError_t sweep_fp64(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+                     int grid, hipStream_t st)
+{
+    return sweep_t<double, double>(which, ty, ny, dm, shots, items, nitems, grid, st);
+}
+
+}  // namespace sg

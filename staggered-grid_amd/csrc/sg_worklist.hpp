@@ -9,10 +9,11 @@
 //    pieces then split into near-equal parts of at most TY rows / zc planes
 //    (z PML slabs: max(zc, min(P, 32)) planes), so damping in y and z is
 //    uniform per box and row/plane.
-// INTERIOR items (no damped column, row or plane) and PML items run
-// different kernel instantiations.  Items of each kind are interleaved so
-// that item position p lands on XCD p % 8 (blocks are dispatched round-robin
-// over the 8 XCDs) while every XCD walks a contiguous band of boxes.
+// Boxes are listed by the set of axes that can damp inside them: interior
+// (none), x, y or z face, edges and corners (several); each list runs its own
+// kernel instantiation.  Each list is interleaved so that item position p
+// lands on XCD p % 8 (blocks are dispatched round-robin over the 8 XCDs)
+// while every XCD walks a contiguous band of boxes.
 #pragma once
 
 #include <algorithm>
@@ -71,11 +72,37 @@ inline int auto_zchunk(const Geom &g, int ty, int target = 8192)
     return std::max(8, std::min(zc, g.uz));
 }
 
+// interleave for the XCD round-robin; pads with shot = -1 to a multiple of 8
+inline std::vector<WorkItem> xcd_order(const std::vector<WorkItem> &seq)
+{
+    const int n = (int)seq.size();
+    const int per = (n + 7) / 8;
+    std::vector<WorkItem> tab((size_t)per * 8, WorkItem{-1, 0, 0, 0, 0, 0, 0});
+    for (int s = 0; s < n; ++s) tab[(size_t)(s % per) * 8 + s / per] = seq[s];
+    return tab;
+}
+
+// One list per damping mask (LIST_DM order: interior, x, y, z faces, edges
+// and corners).
+struct WorkLists {
+    std::vector<WorkItem> l[NLIST];
+};
+
+inline int list_of_mask(int m)
+{
+    switch (m) {
+    case 0: return 0;
+    case 1: return 1;
+    case 2: return 2;
+    case 4: return 3;
+    default: return 4;
+    }
+}
+
 // Interior boxes use y pieces of at most `ty` rows, PML boxes of at most
-// `pty`; both sets of y cuts include the PML planes, so the two lists
-// partition the updated range.
-inline void build_worklists(const Geom &g, int shot, int ty, int pty, int zc, std::vector<WorkItem> &interior,
-                            std::vector<WorkItem> &pml)
+// `pty`; both sets of y cuts include the PML planes, so the lists partition
+// the updated range.
+inline void build_worklists(const Geom &g, int shot, int ty, int pty, int zc, WorkLists &wl)
 {
     const std::vector<Seg> xs = x_tiles(g.nx, g.pml);
     const std::vector<Seg> ysi = axis_segments(g.ny, g.pml, ty, ty);
@@ -86,21 +113,26 @@ inline void build_worklists(const Geom &g, int shot, int ty, int pty, int zc, st
     for (const Seg &z : zs) {
         for (const Seg &y : ysi)
             for (const Seg &x : xs)
-                if (!(x.pml || y.pml || z.pml)) interior.push_back(WorkItem{shot, x.a, x.b, y.a, y.b, z.a, z.b});
+                if (!(x.pml || y.pml || z.pml)) wl.l[0].push_back(WorkItem{shot, x.a, x.b, y.a, y.b, z.a, z.b});
         for (const Seg &y : ysp)
-            for (const Seg &x : xs)
-                if (x.pml || y.pml || z.pml) pml.push_back(WorkItem{shot, x.a, x.b, y.a, y.b, z.a, z.b});
+            for (const Seg &x : xs) {
+                const int m = (x.pml ? 1 : 0) | (y.pml ? 2 : 0) | (z.pml ? 4 : 0);
+                if (m) wl.l[list_of_mask(m)].push_back(WorkItem{shot, x.a, x.b, y.a, y.b, z.a, z.b});
+            }
     }
 }
 
-// interleave for the XCD round-robin; pads with shot = -1 to a multiple of 8
-inline std::vector<WorkItem> xcd_order(const std::vector<WorkItem> &seq)
+// XCD-ordered lists back to back; list l occupies [off[l], off[l+1])
+inline std::vector<WorkItem> pack_worklists(const WorkLists &wl, int off[NLIST + 1])
 {
-    const int n = (int)seq.size();
-    const int per = (n + 7) / 8;
-    std::vector<WorkItem> tab((size_t)per * 8, WorkItem{-1, 0, 0, 0, 0, 0, 0});
-    for (int s = 0; s < n; ++s) tab[(size_t)(s % per) * 8 + s / per] = seq[s];
-    return tab;
+    std::vector<WorkItem> all;
+    off[0] = 0;
+    for (int l = 0; l < NLIST; ++l) {
+        const std::vector<WorkItem> t = xcd_order(wl.l[l]);
+        all.insert(all.end(), t.begin(), t.end());
+        off[l + 1] = (int)all.size();
+    }
+    return all;
 }
 
 }  // namespace sg
