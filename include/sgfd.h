@@ -169,6 +169,11 @@ int sg_get_field_f64(sg_solver *s, int field, double *out);
  * out[1] = sum of the six stress components squared.  Used to check that
  * the PML drains energy after the source stops. */
 int sg_energy(sg_solver *s, double out[2]);
+/* Largest |value| of each wavefield (vx vy vz sxx syy szz sxy sxz syz),
+ * unscaled.  Run an FP32 solver for a pilot interval and read this to pick
+ * the power-of-two storage scales of an FP16/BF16 solver
+ * (sg_set_field_scales).  No counterpart in the reference. */
+int sg_field_maxabs(sg_solver *s, float out[9]);
 
 /* ---- timing and tuning -------------------------------------------------------- */
 

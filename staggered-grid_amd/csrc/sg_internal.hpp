@@ -169,6 +169,9 @@ hipError_t launch_gather(int prec, const void *shot, const long long *idx, const
                          int ncomp, int interval, long long capacity, void *out, const int *d_step,
                          hipStream_t st);
 hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st);
+// largest |value| per wavefield (float bits, atomicMax into out9[9], which
+// the caller zeroes)
+hipError_t launch_maxabs(int prec, const void *shot, unsigned *out9, hipStream_t st);
 hipError_t launch_copy(const float *src, float *dst, long long n, hipStream_t st);
 hipError_t launch_fill_zero(void *p, long long bytes, hipStream_t st);
 hipError_t launch_inject_batch(const void *shots_dev, const AuxDesc *aux, int nshots, hipStream_t st);

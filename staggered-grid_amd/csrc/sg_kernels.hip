@@ -293,6 +293,32 @@ __global__ void k_gather_batch(const Shot<float, float> *shots, const AuxDesc *a
 // ---------------------------------------------------------------------------
 // FP64 energy over the updated region
 // ---------------------------------------------------------------------------
+// Largest |value| of each of the nine wavefields (unscaled), as float bits
+// (non-negative floats order like unsigned integers).
+template <class T, class C>
+__global__ __launch_bounds__(256) void k_maxabs(const Shot<T, C> s, unsigned *out9)
+{
+    const long long n = (long long)s.g.nx * s.g.ny * s.g.nz;
+    float m[NFIELD];
+#pragma unroll
+    for (int f = 0; f < NFIELD; ++f) m[f] = 0.0f;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int f = 0; f < NFIELD; ++f) m[f] = fmaxf(m[f], fabsf((float)Cvt<T, C>::ld(s.f[f][t], s.finv[f])));
+    }
+    __shared__ float red[NFIELD][256];
+#pragma unroll
+    for (int f = 0; f < NFIELD; ++f) red[f][threadIdx.x] = m[f];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int f = 0; f < NFIELD; ++f) red[f][threadIdx.x] = fmaxf(red[f][threadIdx.x], red[f][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < NFIELD) atomicMax(&out9[threadIdx.x], __float_as_uint(red[threadIdx.x][0]));
+}
+
 template <class T, class C>
 __global__ __launch_bounds__(256) void k_energy(const Shot<T, C> s, double *out)
 {
@@ -468,6 +494,19 @@ static hipError_t energy_t(const void *shot, double *out2, hipStream_t st)
 hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st)
 {
     SG_DISPATCH(prec, energy_t, shot, out2, st)
+}
+
+template <class T, class C>
+static hipError_t maxabs_t(const void *shot, unsigned *out9, hipStream_t st)
+{
+    const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
+    hipLaunchKernelGGL((k_maxabs<T, C>), dim3(1024), dim3(256), 0, st, s, out9);
+    return hipGetLastError();
+}
+
+hipError_t launch_maxabs(int prec, const void *shot, unsigned *out9, hipStream_t st)
+{
+    SG_DISPATCH(prec, maxabs_t, shot, out9, st)
 }
 
 hipError_t launch_inject_batch(const void *shots_dev, const AuxDesc *aux, int nshots, hipStream_t st)

@@ -411,7 +411,7 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     SG_TRY(hipMemsetAsync(s->saved, 0, 4 * sT, s->stream));
     SG_TRY(hipMalloc(&s->d_step, sizeof(int)));
     SG_TRY(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
-    SG_TRY(hipMalloc(&s->d_energy, 2 * sizeof(double)));
+    SG_TRY(hipMalloc(&s->d_energy, 2 * sizeof(double) + NFIELD * sizeof(unsigned)));
     SG_TRY(hipMalloc(&s->d_shot, sizeof(Shot<double, double>)));
     SG_TRY(hipStreamSynchronize(s->stream));
 #undef SG_TRY
@@ -750,6 +750,21 @@ int sg_energy(sg_solver *s, double out[2])
     SG_HIP(launch_energy(s->prec, s->shot, s->d_energy, s->stream));
     SG_HIP(hipMemcpyAsync(out, s->d_energy, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     SG_HIP(hipStreamSynchronize(s->stream));
+    return SG_OK;
+}
+
+int sg_field_maxabs(sg_solver *s, float out[9])
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!out) return fail(SG_ERR_INVALID, "sg_field_maxabs: out is NULL");
+    SG_HIP(hipSetDevice(s->device));
+    unsigned *d = reinterpret_cast<unsigned *>(s->d_energy + 2);
+    unsigned h[NFIELD];
+    SG_HIP(launch_fill_zero(d, NFIELD * sizeof(unsigned), s->stream));
+    SG_HIP(launch_maxabs(s->prec, s->shot, d, s->stream));
+    SG_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    for (int f = 0; f < NFIELD; ++f) std::memcpy(&out[f], &h[f], sizeof(float));
     return SG_OK;
 }
 

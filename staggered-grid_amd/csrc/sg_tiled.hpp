@@ -126,6 +126,36 @@ __device__ __forceinline__ void bst(T *p, unsigned boff, T v)
     *(gT *)((gc *)p + boff) = v;
 }
 
+// Streaming (nontemporal) variants for the read-modify-write operands of a
+// sweep, which no other box reads: they bypass L2 retention so the planes
+// shared through halos stay cached.  Enabled with SG_NT=1.
+#ifndef SG_NT
+#define SG_NT 0
+#endif
+template <class T>
+__device__ __forceinline__ T bld_nt(const T *p, unsigned boff)
+{
+#if SG_NT
+    typedef __attribute__((address_space(1))) const char gcc;
+    typedef __attribute__((address_space(1))) const T gT;
+    return __builtin_nontemporal_load((gT *)((gcc *)p + boff));
+#else
+    return bld(p, boff);
+#endif
+}
+
+template <class T>
+__device__ __forceinline__ void bst_nt(T *p, unsigned boff, T v)
+{
+#if SG_NT
+    typedef __attribute__((address_space(1))) char gc;
+    typedef __attribute__((address_space(1))) T gT;
+    __builtin_nontemporal_store(v, (gT *)((gc *)p + boff));
+#else
+    bst(p, boff, v);
+#endif
+}
+
 template <>
 __device__ __forceinline__ bf16 bld<bf16>(const bf16 *p, unsigned boff)
 {
@@ -138,6 +168,18 @@ __device__ __forceinline__ void bst<bf16>(bf16 *p, unsigned boff, bf16 v)
 {
     typedef __attribute__((address_space(1))) char gc;
     *(__attribute__((address_space(1))) uint16_t *)((gc *)p + boff) = v.bits;
+}
+
+template <>
+__device__ __forceinline__ bf16 bld_nt<bf16>(const bf16 *p, unsigned boff)
+{
+    return bld<bf16>(p, boff);
+}
+
+template <>
+__device__ __forceinline__ void bst_nt<bf16>(bf16 *p, unsigned boff, bf16 v)
+{
+    bst<bf16>(p, boff, v);
 }
 
 template <>
@@ -392,7 +434,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 #pragma unroll
         for (int r = 0; r < NY; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) vold[c][r] = Cvt<T, C>::ld(bld(pv[c], t.ro[r]), s.finv[VX + c]);
+            for (int c = 0; c < 3; ++c) vold[c][r] = Cvt<T, C>::ld(bld_nt(pv[c], t.ro[r]), s.finv[VX + c]);
         bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1, so[NY][3][3];
         unsigned sb[NY];
@@ -482,7 +524,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
             }
             if (t.rup[r]) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) bst(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
+                for (int c = 0; c < 3; ++c) bst_nt(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
             }
             if (PML && any[r]) {
                 if (DX && px) {
@@ -664,9 +706,9 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         C old[6][NY];
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            lam[r] = Cvt<T, C>::ld(bld(pl, t.ro[r]), il);
+            lam[r] = Cvt<T, C>::ld(bld_nt(pl, t.ro[r]), il);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) old[c][r] = Cvt<T, C>::ld(bld(ps[c], t.ro[r]), s.finv[SXX + c]);
+            for (int c = 0; c < 6; ++c) old[c][r] = Cvt<T, C>::ld(bld_nt(ps[c], t.ro[r]), s.finv[SXX + c]);
         }
         if (k == src_k && s.src_saved) {
             // TRANSIENT source: the stress update starts from sigma before injection
@@ -810,7 +852,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
             }
             if (t.rup[r]) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) bst(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
+                for (int c = 0; c < 6; ++c) bst_nt(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
             }
             if (PML && any[r]) {
                 const unsigned o = sb[r];

@@ -85,6 +85,7 @@ API = {
     "sg_get_field": (ctypes.c_int, [_VP, ctypes.c_int, _FP]),
     "sg_get_field_f64": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     "sg_energy": (ctypes.c_int, [_VP, _DP]),
+    "sg_field_maxabs": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "sg_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP]),
     "sg_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_set_tile": (ctypes.c_int, [_VP, _TP]),
@@ -325,6 +326,12 @@ class Solver:
         _check(lib().sg_energy(self._h, out.ctypes.data_as(_DP)))
         return out
 
+    def field_maxabs(self):
+        """Largest |value| of each of the nine wavefields (unscaled)."""
+        out = np.zeros(9, np.float32)
+        _check(lib().sg_field_maxabs(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
 
 class Batch:
     """Several FP32 solvers advanced by one launch per half-step (sg_batch_*).
@@ -435,3 +442,13 @@ class ElasticWaveEquation:
 
     def close(self):
         self.solver.close()
+
+
+def scales_for_maxabs(maxabs, headroom_log2=3, target_log2=14):
+    """Power-of-two storage scales that map each field's largest |value| to
+    at most 2^(target_log2 - headroom_log2) (fields that stayed 0 get 1)."""
+    out = np.ones(9, np.float32)
+    for f, m in enumerate(np.asarray(maxabs, np.float64)):
+        if m > 0 and np.isfinite(m):
+            out[f] = np.float32(2.0 ** (target_log2 - headroom_log2 - int(np.ceil(np.log2(m)))))
+    return out

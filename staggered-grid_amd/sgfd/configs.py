@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Solver, damping_profile, diff_coef, gaussian_field, ricker
+from . import Solver, damping_profile, diff_coef, scales_for_maxabs, gaussian_field, ricker
 
 HALO = 5
 
@@ -41,6 +41,8 @@ class Workload:
     coef: np.ndarray = field(default_factory=lambda: diff_coef("exact"))
     vp_max: float = 0.0
     description: str = ""
+    f0: float = 0.0
+    _scales: object = None
 
     @property
     def points(self) -> int:
@@ -55,7 +57,35 @@ class Workload:
         s = math.sqrt(1 / self.dx ** 2 + 1 / self.dy ** 2 + 1 / self.dz ** 2)
         return self.dt * self.vp_max * s * float(np.sum(np.abs(self.coef)))
 
-    def solver(self, precision="fp32", kernel="tiled", source_mode="transient", time_length=None):
+    def pilot_scales(self, steps=None, checks=8):
+        """FP16/BF16 storage scales from an FP32 pilot run: the largest
+        |value| of each field over the first `steps` steps (default 2.5
+        source periods, which covers the source peak), mapped below 2^11."""
+        if steps is None:
+            f0 = self.f0 if self.f0 > 0 else 15.0
+            steps = int(np.ceil(2.5 / (f0 * self.dt)))
+        steps = max(1, min(steps, self.nt))
+        s = self.solver("fp32", time_length=steps)
+        try:
+            m = np.zeros(9, np.float32)
+            done = 0
+            for c in range(checks):
+                n = (steps * (c + 1)) // checks - done
+                if n > 0:
+                    s.step(n)
+                    done += n
+                m = np.maximum(m, s.field_maxabs())
+        finally:
+            s.close()
+        return scales_for_maxabs(m)
+
+    def solver(self, precision="fp32", kernel="tiled", source_mode="transient", time_length=None, scales=None):
+        """A solver for this workload.  FP16/BF16 storage scales come from
+        `scales` or, if None, from an FP32 pilot run (cached)."""
+        if precision in ("fp16", "bf16") and scales is None:
+            if self._scales is None:
+                self._scales = self.pilot_scales()
+            scales = self._scales
         s = Solver(self.nx, self.ny, self.nz, self.pml, self.dt, self.dx, self.dy, self.dz, coef=self.coef,
                    time_length=self.nt if time_length is None else time_length, precision=precision,
                    kernel=kernel, source_mode=source_mode)
@@ -65,6 +95,8 @@ class Workload:
             s.set_source(*self.src, self.wavelet)
             if len(self.stations):
                 s.set_stations(self.stations, comps=self.comps, interval=self.interval)
+            if scales is not None and precision in ("fp16", "bf16"):
+                s.set_field_scales(scales)
         except Exception:
             s.close()
             raise
@@ -106,7 +138,7 @@ def plain100() -> Workload:
     st = np.array([[20 + 3 * r, 50, 20] for r in range(20)], np.int32)
     return Workload("plain100", n, n, n, pml, 1e-3, 10.0, 10.0, 10.0, nt, b, lam, mu, (50, 50, 50),
                     ricker(15.0, 1e-3, nt, "none"), st, damp_v=3000.0, vp_max=3000.0,
-                    description="100^3 P10 homogeneous, 500 steps")
+                    description="100^3 P10 homogeneous, 500 steps", f0=15.0)
 
 
 def random256(seed=1234, nt=1000) -> Workload:
@@ -120,7 +152,7 @@ def random256(seed=1234, nt=1000) -> Workload:
     st = _surface_grid(n, n, HALO + pml, pml, 64)
     return Workload("random256", n, n, n, pml, 1e-3, 10.0, 10.0, 10.0, nt, b, lam, mu, (128, 128, 40),
                     ricker(15.0, 1e-3, nt, "none"), st, damp_v=3000.0, vp_max=float(vp.max()),
-                    description="256^3 P16 random medium, 1000 steps")
+                    description="256^3 P16 random medium, 1000 steps", f0=15.0)
 
 
 def layered512(seed=2024, nt=2000, nz=384) -> Workload:
@@ -138,7 +170,7 @@ def layered512(seed=2024, nt=2000, nz=384) -> Workload:
     st = _surface_grid(nx, ny, HALO + pml, pml, 64)
     return Workload("layered512", nx, ny, nz, pml, 8e-4, 10.0, 10.0, 10.0, nt, b, lam, mu, (256, 256, 40),
                     ricker(12.0, 8e-4, nt, "none"), st, damp_v=vmax, vp_max=vmax,
-                    description="512x512x384 P20 4-layer + 5% random, 2000 steps")
+                    description="512x512x384 P20 4-layer + 5% random, 2000 steps", f0=12.0)
 
 
 def large1024(seed=4096, nt=1500) -> Workload:
@@ -157,7 +189,7 @@ def large1024(seed=4096, nt=1500) -> Workload:
     st = _surface_grid(nx, ny, HALO + pml, pml, 256)
     return Workload("large1024", nx, ny, nz, pml, 1e-3, 12.5, 12.5, 12.5, nt, b, lam, mu, (512, 512, 60),
                     ricker(12.0, 1e-3, nt, "none"), st, interval=4, damp_v=vmax, vp_max=vmax,
-                    description="1024x1024x640 P24 layered + random, 1500 steps")
+                    description="1024x1024x640 P24 layered + random, 1500 steps", f0=12.0)
 
 
 def batched_shots(seed=77, nshots=32, nt=800):
@@ -181,7 +213,7 @@ def batched_shots(seed=77, nshots=32, nt=800):
         st = _surface_grid(nx, ny, HALO + pml, pml, 16)
         shots.append(Workload(f"shot{s}", nx, ny, nz, pml, 1e-3, 10.0, 10.0, 10.0, nt, b, lam, mu, src,
                               ricker(f0, 1e-3, nt, "none"), st, damp_v=float(vp.max()), vp_max=float(vp.max()),
-                              description=f"shot {s}: {nx}x{ny}x{nz}"))
+                              description=f"shot {s}: {nx}x{ny}x{nz}", f0=f0))
     return shots
 
 
