@@ -40,7 +40,7 @@ struct sg_batch {
     WorkItem *d_items = nullptr;
     int off[NLIST + 1] = {0, 0, 0, 0, 0, 0};  // work-lists back to back (each padded to a multiple of 8)
     int grid = 0;
-    Tile tile{16, 1, 8, 1, 16};
+    Tile tile{4, 1, 4, 1, 24};
     int max_gather = 0;
     bool use_graph = true;
     bool graph_valid = false;

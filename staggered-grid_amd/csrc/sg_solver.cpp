@@ -94,7 +94,7 @@ struct sg_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     Geom g{};
-    Tile tile{16, 1, 8, 1, 0};
+    Tile tile{4, 1, 4, 1, 0};
 
     void *fields[NFIELD] = {};
     void *model[3] = {};

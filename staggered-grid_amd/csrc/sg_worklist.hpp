@@ -62,14 +62,15 @@ inline std::vector<Seg> x_tiles(int n, int pml)
     return s;
 }
 
-// z chunk length giving about `target` work items for the shot (>= 8 planes)
-inline int auto_zchunk(const Geom &g, int ty, int target = 8192)
+// z chunk length: 24 planes (ten planes of z-queue warm-up per chunk against
+// load balance, measured best at 512x512x384), shortened down to 8 while the
+// shot would give fewer than `target` boxes
+inline int auto_zchunk(const Geom &g, int ty, int target = 2048)
 {
     const long long cols = (long long)((g.ux + 63) / 64) * ((g.uy + ty - 1) / ty);
-    long long nbz = (target + cols - 1) / cols;
-    if (nbz < 1) nbz = 1;
-    int zc = (int)((g.uz + nbz - 1) / nbz);
-    return std::max(8, std::min(zc, g.uz));
+    int zc = 24;
+    while (zc > 8 && cols * ((g.uz + zc - 1) / zc) < target) zc -= 4;
+    return std::max(1, std::min(zc, g.uz));
 }
 
 // interleave for the XCD round-robin; pads with shot = -1 to a multiple of 8

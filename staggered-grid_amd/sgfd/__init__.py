@@ -277,7 +277,7 @@ class Solver:
     def set_graph(self, enable=True):
         _check(lib().sg_set_graph(self._h, 1 if enable else 0))
 
-    def set_tile(self, ty=16, ny=1, pty=8, pny=1, zchunk=0):
+    def set_tile(self, ty=4, ny=1, pty=4, pny=1, zchunk=0):
         """Box shapes: interior (ty rows per box, ny rows per thread), PML
         (pty, pny), zchunk planes per box (0 = auto)."""
         _check(lib().sg_set_tile(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk))))
@@ -344,7 +344,7 @@ class Batch:
         _check(lib().sg_batch_create(arr, len(self.solvers), ctypes.byref(h)))
         self._h = h
 
-    def set_launch(self, ty=16, ny=1, pty=8, pny=1, zchunk=16, grid=0, graph=True):
+    def set_launch(self, ty=4, ny=1, pty=4, pny=1, zchunk=24, grid=0, graph=True):
         _check(lib().sg_batch_set_launch(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk)), grid,
                                          1 if graph else 0))
 
