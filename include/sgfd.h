@@ -75,6 +75,12 @@ typedef enum sg_field {
 /* Kernel variants for the two half-step sweeps. */
 typedef enum sg_kernel { SG_KERNEL_TILED = 0, SG_KERNEL_SIMPLE = 1 } sg_kernel;
 
+/* Rounding of 16-bit wavefield stores (FP16/BF16 solvers; FP32/FP64 ignore
+ * it): nearest even, or stochastic (unbiased; the random bits come from a
+ * hash of point, step and field, so results are reproducible and independent
+ * of launch shape).  No counterpart in the reference (FP32 only). */
+typedef enum sg_rounding { SG_ROUND_NEAREST = 0, SG_ROUND_STOCHASTIC = 1 } sg_rounding;
+
 /* Damping profile formulas for sg_damping_profile(). */
 typedef enum sg_damping_kind {
     SG_DAMP_REFERENCE = 0, /* log10(1/R)*tau*V/(P*h)*(dist/P)^2  (CPUPML/ElasticMacro.h:21-23) */
@@ -141,6 +147,17 @@ int sg_set_stations(sg_solver *s, const int *xyz, int n, unsigned comp_mask, int
  * [2^13, 2^14)).  sg_get_field_scales returns the 9 field scales followed by
  * the buoyancy, lambda and mu scales. */
 int sg_set_field_scales(sg_solver *s, const float scales[9]);
+/* Rounding mode of 16-bit wavefield stores (sg_rounding).  Fields with a
+ * compensation array always round to nearest. */
+int sg_set_rounding(sg_solver *s, int mode);
+/* Compensated time update for 16-bit storage (FP16/BF16 solvers, tiled
+ * sweeps): for each field whose bit (1 << sg_field) is set, a second 16-bit
+ * array holds the rounding residual of the last store, added back at the
+ * next update of that point, so rounding errors do not accumulate over time
+ * steps (stencil neighbours still read the 16-bit value).  Costs two more
+ * 16-bit accesses per point per step for each field.  Call before stepping;
+ * 0 turns it off.  No counterpart in the reference. */
+int sg_set_compensation(sg_solver *s, unsigned field_mask);
 int sg_get_field_scales(const sg_solver *s, float out[12]);
 
 /* ---- stepping --------------------------------------------------------------- */

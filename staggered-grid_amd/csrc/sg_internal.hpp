@@ -101,7 +101,17 @@ struct Shot {
     float fscale[NFIELD];
     float finv[NFIELD];
     float bscale, lscale, mscale;  // model storage scales (value = stored / scale)
+    int round_mode;                // 16-bit storage: 0 nearest even, 1 stochastic
+    const int *d_step;             // steps started (advanced by the source kernel)
+    // 16-bit storage, compensated time update: per field (or null) the
+    // rounding residual of the last store, stored as residual * cscale
+    T *comp[NFIELD];
+    float cscale[NFIELD], cinv[NFIELD];
 };
+
+// residual scale of compensation arrays relative to the field scale: the
+// residual of a 16-bit store is below half an ulp, 2^-11 of the value
+constexpr float COMP_GAIN = 1024.0f;
 
 // Box shapes of the tiled sweeps (block = 64 x ty/ny threads).
 struct Tile {

@@ -27,10 +27,13 @@ namespace sg {
 // simple sweeps
 // ---------------------------------------------------------------------------
 
+// field value at o (with its compensation residual when there is one)
 template <class T, class C>
 __device__ __forceinline__ C ldf(const Shot<T, C> &s, int f, long long o)
 {
-    return Cvt<T, C>::ld(s.f[f][o], s.finv[f]);
+    C v = Cvt<T, C>::ld(s.f[f][o], s.finv[f]);
+    if (sizeof(T) == 2 && s.comp[f]) v += Cvt<T, C>::ld(s.comp[f][o], s.cinv[f]);
+    return v;
 }
 
 template <class T, class C>
@@ -59,6 +62,16 @@ __device__ __forceinline__ C dbwd_g(const Shot<T, C> &s, int f, long long o, lon
     r += c[3] * (L(o + 3 * st) - L(o - 4 * st));
     r += c[4] * (L(o + 4 * st) - L(o - 5 * st));
     return r;
+}
+
+// store of an updated field value (stochastic rounding when enabled for
+// 16-bit storage, keyed by point, step and field)
+template <class T, class C>
+__device__ __forceinline__ T store_val(const Shot<T, C> &s, int f, long long o, C v)
+{
+    if (sizeof(T) == 2 && s.round_mode)
+        return CvtSR<T, C>::st(v, s.fscale[f], round_key((uint32_t)o, (uint32_t)*s.d_step, (uint32_t)f));
+    return Cvt<T, C>::st(v, s.fscale[f]);
 }
 
 template <class T, class C>
@@ -96,7 +109,7 @@ __global__ __launch_bounds__(256) void k_velocity_simple(const Shot<T, C> s)
         for (int c = 0; c < 3; ++c) {
             C v = ldf(s, VX + c, o);
             v += db[c] * (d[c][0] + d[c][1] + d[c][2]);
-            s.f[VX + c][o] = Cvt<T, C>::st(v, s.fscale[VX + c]);
+            s.f[VX + c][o] = store_val(s, VX + c, o, v);
         }
         return;
     }
@@ -109,7 +122,7 @@ __global__ __launch_bounds__(256) void k_velocity_simple(const Shot<T, C> s)
         const C inc[3] = {db[c] * d[c][0], db[c] * d[c][1], db[c] * d[c][2]};
         C *const sp[3] = {s.sv[3 * c], s.sv[3 * c + 1], s.sv[3 * c + 2]};
         const C v = pml_update<C, 3>(ldf(s, VX + c, o), inc, dm, sp, slot, A, B);
-        s.f[VX + c][o] = Cvt<T, C>::st(v, s.fscale[VX + c]);
+        s.f[VX + c][o] = store_val(s, VX + c, o, v);
     }
 }
 
@@ -205,7 +218,7 @@ __global__ __launch_bounds__(256) void k_stress_simple(const Shot<T, C> s)
         }
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s.f[SXX + c][o] = Cvt<T, C>::st(nw[c], s.fscale[SXX + c]);
+    for (int c = 0; c < 6; ++c) s.f[SXX + c][o] = store_val(s, SXX + c, o, nw[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -224,13 +237,22 @@ __global__ void k_inject(const Shot<T, C> s, const double *amp_d, const C *amp_a
     *d_step = it + 1;
     if (s.src < 0) return;
     for (int c = 0; c < 3; ++c) {
-        T *p = s.f[SXX + c] + s.src;
-        const C cur = Cvt<T, C>::ld(*p, s.finv[SXX + c]);
+        const int f = SXX + c;
+        T *p = s.f[f] + s.src;
+        T *pc = (sizeof(T) == 2 && s.comp[f]) ? s.comp[f] + s.src : nullptr;
+        const C cur = ldf(s, f, s.src);
+        C nv = cur;
         if (accumulate) {
-            if (it < namp) *p = Cvt<T, C>::st(cur + amp_acc[it], s.fscale[SXX + c]);
+            if (it < namp) nv = cur + amp_acc[it];
         } else {
             saved[c] = *p;
-            if (it < namp) *p = Cvt<T, C>::st((C)((double)cur + amp_d[it]), s.fscale[SXX + c]);
+            if (pc) saved[3 + c] = *pc;
+            if (it < namp) nv = (C)((double)cur + amp_d[it]);
+        }
+        if (it < namp) {
+            const T q = Cvt<T, C>::st(nv, s.fscale[f]);
+            *p = q;
+            if (pc) *pc = Cvt<T, C>::st(nv - Cvt<T, C>::ld(q, s.finv[f]), s.cscale[f]);
         }
     }
 }
@@ -250,7 +272,7 @@ __global__ void k_gather(const Shot<T, C> s, const long long *idx, const int *co
     if (t >= nst * ncomp) return;
     const int st = t / ncomp, q = t - st * ncomp;
     const int f = comp[q];
-    out[row * (long long)nst * ncomp + t] = Cvt<T, C>::ld(s.f[f][idx[st]], s.finv[f]);
+    out[row * (long long)nst * ncomp + t] = ldf(s, f, idx[st]);
 }
 
 // batched FP32 forms of the two kernels above: one shot per thread / per grid row

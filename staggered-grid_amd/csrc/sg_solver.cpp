@@ -114,6 +114,9 @@ struct sg_solver {
     long long st_cap = 0;
 
     double *d_energy = nullptr;
+    int round_mode = SG_ROUND_NEAREST;
+    void *comp[NFIELD] = {};  // compensation arrays (16-bit storage, see sg_set_compensation)
+    unsigned comp_mask = 0;
     long long steps_done = 0;
     std::vector<double> damp[3];
 
@@ -175,6 +178,13 @@ void fill_shot(sg_solver *s)
     h.bscale = s->bscale;
     h.lscale = s->lscale;
     h.mscale = s->mscale;
+    h.round_mode = s->round_mode;
+    h.d_step = s->d_step;
+    for (int f = 0; f < NFIELD; ++f) {
+        h.comp[f] = static_cast<T *>(s->comp[f]);
+        h.cscale[f] = s->fscale[f] * COMP_GAIN;
+        h.cinv[f] = 1.0f / (s->fscale[f] * COMP_GAIN);
+    }
 }
 
 size_t shot_size(int prec)
@@ -310,8 +320,10 @@ void free_all(sg_solver *s)
                     s->st_idx, s->st_comp, s->st_out, s->d_energy, s->d_shot, s->d_wl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    for (int f = 0; f < NFIELD; ++f)
+    for (int f = 0; f < NFIELD; ++f) {
         if (s->fields[f]) (void)hipFree(s->fields[f]);
+        if (s->comp[f]) (void)hipFree(s->comp[f]);
+    }
     for (int m = 0; m < 3; ++m)
         if (s->model[m]) (void)hipFree(s->model[m]);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -407,8 +419,8 @@ int sg_create(const sg_config *cfg, sg_solver **out)
         SG_TRY(hipMemsetAsync(s->splits, 0, (size_t)g.nslots * 24 * sC, s->stream));
     }
     SG_TRY(hipMalloc(&s->tables, 2 * (size_t)(c.nx + c.ny + c.nz) * sC));
-    SG_TRY(hipMalloc(&s->saved, 4 * sT));
-    SG_TRY(hipMemsetAsync(s->saved, 0, 4 * sT, s->stream));
+    SG_TRY(hipMalloc(&s->saved, 8 * sT));
+    SG_TRY(hipMemsetAsync(s->saved, 0, 8 * sT, s->stream));
     SG_TRY(hipMalloc(&s->d_step, sizeof(int)));
     SG_TRY(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
     SG_TRY(hipMalloc(&s->d_energy, 2 * sizeof(double) + NFIELD * sizeof(unsigned)));
@@ -661,7 +673,10 @@ int sg_reset(sg_solver *s)
     SG_HIP(hipSetDevice(s->device));
     SG_HIP(hipStreamSynchronize(s->stream));
     const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz, sT = storage_size(s->prec);
-    for (int f = 0; f < NFIELD; ++f) SG_HIP(hipMemsetAsync(s->fields[f], 0, N * sT, s->stream));
+    for (int f = 0; f < NFIELD; ++f) {
+        SG_HIP(hipMemsetAsync(s->fields[f], 0, N * sT, s->stream));
+        if (s->comp[f]) SG_HIP(hipMemsetAsync(s->comp[f], 0, N * sT, s->stream));
+    }
     if (s->splits) SG_HIP(hipMemsetAsync(s->splits, 0, (size_t)s->g.nslots * 24 * compute_size(s->prec), s->stream));
     SG_HIP(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
     if (s->st_out)
@@ -717,11 +732,17 @@ static int get_field_any(sg_solver *s, int field, float *outf, double *outd)
             for (size_t i = 0; i < N; ++i) outf[i] = (float)t[i];
         }
     } else {
-        std::vector<uint16_t> t(N);
+        std::vector<uint16_t> t(N), c;
         SG_HIP(hipMemcpy(t.data(), s->fields[field], N * 2, hipMemcpyDeviceToHost));
-        const float inv = 1.0f / s->fscale[field];
+        if (s->comp[field]) {
+            c.resize(N);
+            SG_HIP(hipMemcpy(c.data(), s->comp[field], N * 2, hipMemcpyDeviceToHost));
+        }
+        const float inv = 1.0f / s->fscale[field], cinv = 1.0f / (s->fscale[field] * COMP_GAIN);
+        auto dec = [&](uint16_t b) { return s->prec == SG_FP16 ? from_half_bits(b) : from_bf16_bits(b); };
         for (size_t i = 0; i < N; ++i) {
-            const float v = (s->prec == SG_FP16 ? from_half_bits(t[i]) : from_bf16_bits(t[i])) * inv;
+            float v = dec(t[i]) * inv;
+            if (!c.empty()) v += dec(c[i]) * cinv;
             if (outf) outf[i] = v;
             else outd[i] = v;
         }
@@ -751,6 +772,48 @@ int sg_energy(sg_solver *s, double out[2])
     SG_HIP(hipMemcpyAsync(out, s->d_energy, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     SG_HIP(hipStreamSynchronize(s->stream));
     return SG_OK;
+}
+
+int sg_set_compensation(sg_solver *s, unsigned field_mask)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (field_mask >> NFIELD) return fail(SG_ERR_INVALID, "sg_set_compensation: mask has bits above the nine fields");
+    if (field_mask && s->prec != SG_FP16 && s->prec != SG_BF16)
+        return fail(SG_ERR_UNSUPPORTED, "sg_set_compensation: only FP16/BF16 storage is compensated");
+    if (field_mask && s->cfg.kernel != SG_KERNEL_TILED)
+        return fail(SG_ERR_UNSUPPORTED, "sg_set_compensation: needs the tiled sweeps");
+    if (s->steps_done != 0) return fail(SG_ERR_STATE, "sg_set_compensation: set it before stepping (or after sg_reset)");
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz, sT = storage_size(s->prec);
+    for (int f = 0; f < NFIELD; ++f) {
+        const bool on = (field_mask >> f) & 1u;
+        if (on && !s->comp[f]) {
+            hipError_t e = hipMalloc(&s->comp[f], N * sT);
+            if (e != hipSuccess) {
+                s->comp[f] = nullptr;
+                return fail(e == hipErrorOutOfMemory ? SG_ERR_NOMEM : SG_ERR_HIP,
+                            std::string("sg_set_compensation: ") + hipGetErrorString(e));
+            }
+        } else if (!on && s->comp[f]) {
+            SG_HIP(hipFree(s->comp[f]));
+            s->comp[f] = nullptr;
+        }
+        if (s->comp[f]) SG_HIP(hipMemsetAsync(s->comp[f], 0, N * sT, s->stream));
+    }
+    SG_HIP(hipStreamSynchronize(s->stream));
+    s->comp_mask = field_mask;
+    s->graph_valid = false;
+    return refresh_shot(s);
+}
+
+int sg_set_rounding(sg_solver *s, int mode)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (mode != SG_ROUND_NEAREST && mode != SG_ROUND_STOCHASTIC)
+        return fail(SG_ERR_INVALID, "sg_set_rounding: mode must be SG_ROUND_NEAREST or SG_ROUND_STOCHASTIC");
+    s->round_mode = mode;
+    return refresh_shot(s);
 }
 
 int sg_field_maxabs(sg_solver *s, float out[9])

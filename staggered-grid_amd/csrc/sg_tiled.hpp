@@ -304,6 +304,20 @@ __device__ __forceinline__ void tile_geo(const Geom &g, int i0, int i1, int j0, 
     t.bxyo = (unsigned)(bxc + byc * sy) * ES;
 }
 
+// Store of an updated value with compensation when field f has a
+// compensation array: the 16-bit value and the residual of its rounding
+// (which the next update adds back); plain rounding otherwise.
+template <class T, class C>
+__device__ __forceinline__ void store_comp(const Shot<T, C> &s, int f, int k, unsigned boff, C v, T *pf)
+{
+    const T q = Cvt<T, C>::st(v, s.fscale[f]);
+    bst_nt(pf, boff, q);
+    if (s.comp[f]) {
+        const C res = v - Cvt<T, C>::ld(q, s.finv[f]);
+        bst_nt(plane(s.comp[f], k, s.g.sz), boff, Cvt<T, C>::st(res, s.cscale[f]));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
@@ -378,6 +392,8 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     const bool py = DY && in_pml(j0, g.ny, g.pml);
     C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
     const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
+    // step number keying stochastic rounding (16-bit storage only)
+    const uint32_t step = (sizeof(T) == 2 && s.round_mode) ? (uint32_t)gld(s.d_step, 0u) : 0u;
     if (DX) {
         A0 = gld(s.dA[0], (unsigned)(t.ic));
         B0 = gld(s.dB[0], (unsigned)(t.ic));
@@ -435,6 +451,16 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         for (int r = 0; r < NY; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) vold[c][r] = Cvt<T, C>::ld(bld_nt(pv[c], t.ro[r]), s.finv[VX + c]);
+        if (sizeof(T) == 2) {
+            // compensated update: the old value is stored value + residual
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (s.comp[VX + c]) {
+                    const T *pc = plane(s.comp[VX + c], k, sz);
+#pragma unroll
+                    for (int r = 0; r < NY; ++r) vold[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[VX + c]);
+                }
+        }
         bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1, so[NY][3][3];
         unsigned sb[NY];
@@ -523,8 +549,19 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 }
             }
             if (t.rup[r]) {
+                if (sizeof(T) == 2 && (!s.round_mode || s.comp[VX] || s.comp[VY] || s.comp[VZ])) {
+                    store_comp<T, C>(s, VX, k, t.ro[r], vn[0], pv[0]);
+                    store_comp<T, C>(s, VY, k, t.ro[r], vn[1], pv[1]);
+                    store_comp<T, C>(s, VZ, k, t.ro[r], vn[2], pv[2]);
+                } else if (sizeof(T) == 2 && s.round_mode) {
+                    const uint32_t pt = t.ro[r] / (unsigned)sizeof(T) + (uint32_t)k * (uint32_t)sz;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) bst_nt(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
+                    for (int c = 0; c < 3; ++c)
+                        bst_nt(pv[c], t.ro[r], CvtSR<T, C>::st(vn[c], s.fscale[VX + c], round_key(pt, step, VX + c)));
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bst_nt(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
+                }
             }
             if (PML && any[r]) {
                 if (DX && px) {
@@ -657,6 +694,8 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     const bool py = DY && in_pml(j0, g.ny, g.pml);
     C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
     const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
+    // step number keying stochastic rounding (16-bit storage only)
+    const uint32_t step = (sizeof(T) == 2 && s.round_mode) ? (uint32_t)gld(s.d_step, 0u) : 0u;
     if (DX) {
         A0 = gld(s.dA[0], (unsigned)(t.ic));
         B0 = gld(s.dB[0], (unsigned)(t.ic));
@@ -710,13 +749,27 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #pragma unroll
             for (int c = 0; c < 6; ++c) old[c][r] = Cvt<T, C>::ld(bld_nt(ps[c], t.ro[r]), s.finv[SXX + c]);
         }
+        if (sizeof(T) == 2) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (s.comp[SXX + c]) {
+                    const T *pc = plane(s.comp[SXX + c], k, sz);
+#pragma unroll
+                    for (int r = 0; r < NY; ++r) old[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[SXX + c]);
+                }
+        }
         if (k == src_k && s.src_saved) {
-            // TRANSIENT source: the stress update starts from sigma before injection
+            // TRANSIENT source: the stress update starts from sigma before
+            // injection (saved: three stored values, then their residuals)
 #pragma unroll
             for (int r = 0; r < NY; ++r)
                 if (t.rup[r] && t.ro[r] == src_o)
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) old[c][r] = Cvt<T, C>::ld(gld(s.src_saved, (unsigned)c), s.finv[SXX + c]);
+                    for (int c = 0; c < 3; ++c) {
+                        old[c][r] = Cvt<T, C>::ld(gld(s.src_saved, (unsigned)c), s.finv[SXX + c]);
+                        if (sizeof(T) == 2 && s.comp[SXX + c])
+                            old[c][r] += Cvt<T, C>::ld(gld(s.src_saved, (unsigned)(3 + c)), s.cinv[SXX + c]);
+                    }
         }
         bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1;
@@ -851,8 +904,19 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 }
             }
             if (t.rup[r]) {
+                if (sizeof(T) == 2 && (!s.round_mode || s.comp[SXX] || s.comp[SYY] || s.comp[SZZ] || s.comp[SXY] ||
+                                       s.comp[SXZ] || s.comp[SYZ])) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) bst_nt(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
+                    for (int c = 0; c < 6; ++c) store_comp<T, C>(s, SXX + c, k, t.ro[r], nw[c], ps[c]);
+                } else if (sizeof(T) == 2 && s.round_mode) {
+                    const uint32_t pt = t.ro[r] / (unsigned)sizeof(T) + (uint32_t)k * (uint32_t)sz;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        bst_nt(ps[c], t.ro[r], CvtSR<T, C>::st(nw[c], s.fscale[SXX + c], round_key(pt, step, SXX + c)));
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) bst_nt(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
+                }
             }
             if (PML && any[r]) {
                 const unsigned o = sb[r];

@@ -86,6 +86,8 @@ API = {
     "sg_get_field_f64": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     "sg_energy": (ctypes.c_int, [_VP, _DP]),
     "sg_field_maxabs": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "sg_set_rounding": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_set_compensation": (ctypes.c_int, [_VP, ctypes.c_uint]),
     "sg_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP]),
     "sg_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_set_tile": (ctypes.c_int, [_VP, _TP]),
@@ -325,6 +327,28 @@ class Solver:
         out = np.zeros(2, np.float64)
         _check(lib().sg_energy(self._h, out.ctypes.data_as(_DP)))
         return out
+
+    def set_rounding(self, mode):
+        """16-bit storage rounding: "nearest" (even) or "stochastic"."""
+        _check(lib().sg_set_rounding(self._h, {"nearest": 0, "stochastic": 1}[mode]))
+
+    def set_compensation(self, fields):
+        """Compensated time update (16-bit storage) for the given fields:
+        an iterable of names ("vx", ..., "syz"), "all", "velocity",
+        "stress", or an int bit mask."""
+        if isinstance(fields, int):
+            mask = fields
+        elif fields == "all":
+            mask = 0x1FF
+        elif fields == "velocity":
+            mask = 0x7
+        elif fields == "stress":
+            mask = 0x1F8
+        else:
+            mask = 0
+            for f in fields:
+                mask |= 1 << FIELDS.index(f)
+        _check(lib().sg_set_compensation(self._h, mask))
 
     def field_maxabs(self):
         """Largest |value| of each of the nine wavefields (unscaled)."""
