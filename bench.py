@@ -29,24 +29,38 @@ def log(msg):
 
 
 def cpu_baseline(wl, planes=60, steps=4):
-    """The C oracle (single thread) on the first `planes` z-planes of the same
-    model (same x/y extent and PML), `steps` time steps."""
+    """CPU leg on a bounded sample of the same workload: the first `planes`
+    z-planes of the model (same x/y extent and PML), `steps` time steps, one
+    thread.  Times the reference's own CPU solver
+    (3DCPUwithPML_3DArray_GridIncludePMLandOrder, built into oracle/_ref by
+    `make -C oracle ref`) when that build is present, else the C restatement
+    in oracle/."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     nz = min(planes, wl.nz)
-    p = O.OracleParams(wl.nx, wl.ny, nz, wl.pml, wl.dt, wl.dx, wl.dy, wl.dz, steps,
-                       (wl.src[0], wl.src[1], min(wl.src[2], nz - 6)), coef=tuple(float(c) for c in wl.coef),
-                       damp_mode=O.DAMP_QUADRATIC, damp_v=wl.damp_v, nthreads=1)
     sl = slice(0, nz)
     b, lam, mu = (np.ascontiguousarray(a[sl]) for a in (wl.b, wl.lam, wl.mu))
+    src = (wl.src[0], wl.src[1], min(wl.src[2], nz // 2))
+    pts = wl.nx * wl.ny * nz
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_baseline")
+    if os.path.exists(exe):
+        P = wl.pml
+        el = O.ref_time("baseline", (wl.nx - 2 * P - 10, wl.ny - 2 * P - 10, nz - 2 * P - 10), P, steps, src,
+                        wl.dx, wl.dy, wl.dz, wl.dt, wl.f0 or 15.0, wl.damp_v, b, lam, mu)
+        return {"value": pts * steps / el / 1e9, "unit": "Gpt-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"reference CPU solver (ElasticWaveEquation3D.cpp, g++ -O2, 1 thread) on z-planes "
+                          f"0..{nz - 1} of the same model ({wl.nx}x{wl.ny}x{nz}, same PML), {steps} steps, "
+                          f"{el:.1f} s timed", "host_cpus": os.cpu_count()}
+    p = O.OracleParams(wl.nx, wl.ny, nz, wl.pml, wl.dt, wl.dx, wl.dy, wl.dz, steps, src,
+                       coef=tuple(float(c) for c in wl.coef), damp_mode=O.DAMP_QUADRATIC, damp_v=wl.damp_v,
+                       nthreads=1)
     w = np.asarray(wl.wavelet[:steps], np.float32)
     t = time.perf_counter()
     O.run(p, b, lam, mu, w, None, fields=False)
     el = time.perf_counter() - t
-    pts = wl.nx * wl.ny * nz
     return {"value": pts * steps / el / 1e9, "unit": "Gpt-updates/s", "cores": 1, "kind": "port",
             "sample": f"oracle/sg_oracle.c (gcc -O2, 1 thread) on z-planes 0..{nz - 1} of the same model "
                       f"({wl.nx}x{wl.ny}x{nz}, same PML), {steps} steps, {el:.1f} s",
@@ -65,6 +79,65 @@ def read_traffic(workload, precision):
     return None
 
 
+def bench_batched(args):
+    """32 uneven shots advanced together (one launch per work-list and
+    half-step, hipGraph replay); value = sum over shots of grid points x
+    steps / time."""
+    import sgfd
+    from sgfd import configs
+
+    log("bench: building 32 shots")
+    t = time.perf_counter()
+    shots = configs.batched_shots(nt=args.warmup + args.steps)
+    log(f"bench: shots ready in {time.perf_counter() - t:.1f} s")
+    solvers = [wl.solver() for wl in shots]
+    try:
+        b = sgfd.Batch(solvers)
+        if args.tile:
+            b.set_launch(*args.tile)
+        b.step(args.warmup)
+        b.synchronize()
+        ms = b.step_timed(args.steps)
+        b.close()
+    finally:
+        for s in solvers:
+            s.close()
+    pts = sum(wl.points for wl in shots)
+    K = args.steps
+    value = pts * K / (ms * 1e-3) / 1e9
+    seq_value = None
+    if args.compare_sequential:
+        log("bench: the same shots one solver at a time")
+        total = 0.0
+        for wl in shots:
+            s = wl.solver()
+            s.step(args.warmup)
+            s.synchronize()
+            _, _, t_ms = s.step_timed(K)
+            total += t_ms
+            s.close()
+        seq_value = pts * K / (total * 1e-3) / 1e9
+    vb = sb = 0.0
+    for wl in shots:
+        s = wl.solver()
+        v, st = s.sweep_bytes()
+        s.close()
+        vb += v
+        sb += st
+    step_frac = (vb + sb) * K / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS
+    out = {"metric": METRIC, "value": value, "unit": "Gpt-updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+           "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "32 shots, nx,ny U{128..320}, nz U{96..256}, P12, 800 steps (batched)",
+                      "points": pts, "parallelism": "single-gpu"},
+           "roofline": {"bound": "hbm", "kernel": "full step (both sweeps, all shots)",
+                        "achieved": (vb + sb) / (ms / K * 1e-3) / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": step_frac, "traffic": None},
+           "sequential_value": seq_value,
+           "cpu_baseline": None}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +151,8 @@ def main():
                          "z planes per box (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
+    ap.add_argument("--compare-sequential", action="store_true",
+                    help="batched32: also time the shots one solver at a time")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -91,6 +166,9 @@ def main():
     if args.copy_calib:
         gbps = sgfd.copy_bandwidth(1 << 30, 20)
         print(json.dumps({"copy_gbps": gbps, "bytes": 1 << 30}))
+        return
+    if args.workload == "batched32":
+        bench_batched(args)
         return
 
     log(f"bench: building workload {args.workload}")

@@ -170,6 +170,26 @@ def ref_available(kind: str = "native") -> bool:
     return os.path.exists(os.path.join(REF_DIR, f"ref_{kind}"))
 
 
+def ref_time(kind: str, interior, pml, nt, src, dx, dy, dz, dt, f0, damp_v, b, lam, mu):
+    """Wall time (s) of `nt` steps of the unmodified reference CPU solver
+    (source -> velocityUpdate -> stressUpdate), timed inside the harness;
+    damping left at zero, which does not change the work per step."""
+    exe = os.path.join(REF_DIR, f"ref_{kind}")
+    nxi, nyi, nzi = interior
+    with tempfile.TemporaryDirectory(dir=REF_DIR) as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<9i", nxi, nyi, nzi, pml, nt, *[int(s) for s in src], 0))
+            f.write(struct.pack("<6f", dx, dy, dz, dt, f0, damp_v))
+            for a in (b, lam, mu):
+                f.write(np.ascontiguousarray(a, np.float32).reshape(-1).tobytes())
+        r = subprocess.run([exe, fin, fout, td, "time"], check=True, capture_output=True, text=True)
+    for line in r.stderr.splitlines():
+        if line.startswith("seconds "):
+            return float(line.split()[1])
+    raise RuntimeError("reference harness printed no timing: " + r.stderr[-500:])
+
+
 def ref_run(kind: str, interior, pml, nt, src, stations, dx, dy, dz, dt, f0, damp_v, b, lam, mu):
     """Run the unmodified reference CPU solver through the harness.
 

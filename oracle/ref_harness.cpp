@@ -24,8 +24,15 @@
  *        damping_x[NX] (line j=k=5), damping_y[NY], damping_z[NZ]
  * The third argument is a scratch directory: calculateDamping() writes its
  * dampingX/Y/Z.txt dumps into the working directory.
+ *
+ * Timing mode (fourth argument "time"): damping stays zero (the update
+ * arithmetic does not depend on its values, so the work per step is the
+ * same), no file is written by the reference, and only the wall time of the
+ * step loop is reported, as "seconds <t>" on stderr; out receives nothing.
  */
+#include <chrono>
 #include <unistd.h>
+#include <string>
 #include <vector>
 
 #include "ElasticMacro.h"
@@ -70,7 +77,7 @@ static bool wr(FILE *f, const void *p, size_t n) { return fwrite(p, 1, n, f) == 
 
 int main(int argc, char **argv)
 {
-    if (argc != 4) {
+    if (argc != 4 && argc != 5) {
         fprintf(stderr, "usage: %s in.bin out.bin scratch_dir\n", argv[0]);
         return 2;
     }
@@ -115,6 +122,18 @@ int main(int argc, char **argv)
     if (chdir(argv[3]) != 0) return 7;
     std::cout.setstate(std::ios_base::failbit); /* the reference prints every step */
 
+    const bool timing = argc > 4 && std::string(argv[4]) == "time";
+    if (timing) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int it = 0; it < nt; ++it) {
+            ewq->loadSeismicSource(it);
+            ewq->velocityUpdate();
+            ewq->stressUpdate();
+        }
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "seconds %.6f\n", sec);
+        return 0;
+    }
     std::vector<float> wavelet(nt), traces((size_t)nt * nst * 9);
     /* The loop of propagate() (ElasticWaveEquation3D.cpp:53-85) without its
      * file output: damping once, then source -> velocity -> stress -> record. */

@@ -83,7 +83,7 @@ int enqueue(sg_batch *b)
     for (int which = 0; which < 2; ++which)
         for (int l = 0; l < NLIST; ++l) {
             const int cnt = b->off[l + 1] - b->off[l];
-            SG_HIP(launch_sweep(which, SG_FP32, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], b->d_shots,
+            SG_HIP(launch_sweep(which, SG_FP32, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], 0, b->d_shots,
                                 b->d_items + b->off[l], cnt, b->grid > 0 ? std::min(b->grid, cnt) : 0, b->stream));
         }
     SG_HIP(launch_gather_batch(b->d_shots, b->d_aux, n, b->max_gather, b->stream));

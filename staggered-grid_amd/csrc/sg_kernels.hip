@@ -469,14 +469,14 @@ static hipError_t inject_t(const void *shot, const double *amp_d, const void *am
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(int which, int prec, int ty, int ny, int dm, const void *shots_dev, const WorkItem *items,
-                        int nitems, int grid, hipStream_t st)
+hipError_t launch_sweep(int which, int prec, int ty, int ny, int dm, int comp, const void *shots_dev,
+                        const WorkItem *items, int nitems, int grid, hipStream_t st)
 {
     switch (prec) {
-    case 0: return sweep_fp32(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
-    case 1: return sweep_fp64(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
-    case 2: return sweep_fp16(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
-    case 3: return sweep_bf16(which, ty, ny, dm, shots_dev, items, nitems, grid, st);
+    case 0: return sweep_fp32(which, ty, ny, dm, comp, shots_dev, items, nitems, grid, st);
+    case 1: return sweep_fp64(which, ty, ny, dm, comp, shots_dev, items, nitems, grid, st);
+    case 2: return sweep_fp16(which, ty, ny, dm, comp, shots_dev, items, nitems, grid, st);
+    case 3: return sweep_bf16(which, ty, ny, dm, comp, shots_dev, items, nitems, grid, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -276,7 +276,7 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
         for (int which = 0; which < 2; ++which) {
             if (which == 1 && e1) SG_HIP(hipEventRecord(e1, s->stream));
             for (int l = 0; l < NLIST; ++l)
-                SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], s->d_shot,
+                SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], s->comp_mask != 0, s->d_shot,
                                     s->d_wl + s->wl_off[l], s->wl_off[l + 1] - s->wl_off[l], 0, s->stream));
         }
     }

@@ -321,7 +321,7 @@ __device__ __forceinline__ void store_comp(const Shot<T, C> &s, int f, int k, un
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int TY, int NY, int DM>
+template <class T, class C, int TY, int NY, int DM, bool CP>
 __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
                                               VelLds<C, TY> &L)
 {
@@ -451,7 +451,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         for (int r = 0; r < NY; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) vold[c][r] = Cvt<T, C>::ld(bld_nt(pv[c], t.ro[r]), s.finv[VX + c]);
-        if (sizeof(T) == 2) {
+        if (CP) {
             // compensated update: the old value is stored value + residual
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -549,7 +549,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 }
             }
             if (t.rup[r]) {
-                if (sizeof(T) == 2 && (!s.round_mode || s.comp[VX] || s.comp[VY] || s.comp[VZ])) {
+                if (CP) {
                     store_comp<T, C>(s, VX, k, t.ro[r], vn[0], pv[0]);
                     store_comp<T, C>(s, VY, k, t.ro[r], vn[1], pv[1]);
                     store_comp<T, C>(s, VZ, k, t.ro[r], vn[2], pv[2]);
@@ -601,7 +601,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 // ---------------------------------------------------------------------------
 // stress sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int TY, int NY, int DM>
+template <class T, class C, int TY, int NY, int DM, bool CP>
 __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
                                               StrLds<C, TY> &L)
 {
@@ -749,7 +749,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #pragma unroll
             for (int c = 0; c < 6; ++c) old[c][r] = Cvt<T, C>::ld(bld_nt(ps[c], t.ro[r]), s.finv[SXX + c]);
         }
-        if (sizeof(T) == 2) {
+        if (CP) {
 #pragma unroll
             for (int c = 0; c < 6; ++c)
                 if (s.comp[SXX + c]) {
@@ -767,7 +767,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         old[c][r] = Cvt<T, C>::ld(gld(s.src_saved, (unsigned)c), s.finv[SXX + c]);
-                        if (sizeof(T) == 2 && s.comp[SXX + c])
+                        if (CP && s.comp[SXX + c])
                             old[c][r] += Cvt<T, C>::ld(gld(s.src_saved, (unsigned)(3 + c)), s.cinv[SXX + c]);
                     }
         }
@@ -904,8 +904,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 }
             }
             if (t.rup[r]) {
-                if (sizeof(T) == 2 && (!s.round_mode || s.comp[SXX] || s.comp[SYY] || s.comp[SZZ] || s.comp[SXY] ||
-                                       s.comp[SXZ] || s.comp[SYZ])) {
+                if (CP) {
 #pragma unroll
                     for (int c = 0; c < 6; ++c) store_comp<T, C>(s, SXX + c, k, t.ro[r], nw[c], ps[c]);
                 } else if (sizeof(T) == 2 && s.round_mode) {
@@ -978,68 +977,81 @@ struct WavesCap {
     static constexpr int v = DM ? SG_PML_WAVES : 1;
 };
 
-template <class T, class C, int TY, int NY, int DM>
+template <class T, class C, int TY, int NY, int DM, bool CP>
 __global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM>::v))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
                                                             const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ VelLds<C, TY> L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) velocity_tile<T, C, TY, NY, DM>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        if (w.shot >= 0) velocity_tile<T, C, TY, NY, DM, CP>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
         __syncthreads();
     }
 }
 
-template <class T, class C, int TY, int NY, int DM>
+template <class T, class C, int TY, int NY, int DM, bool CP>
 __global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM>::v))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
                                                           const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ StrLds<C, TY> L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) stress_tile<T, C, TY, NY, DM>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        if (w.shot >= 0) stress_tile<T, C, TY, NY, DM, CP>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
         __syncthreads();
     }
 }
 
-template <class T, class C, int TY, int NY, int DM>
+template <class T, class C, int TY, int NY, int DM, bool CP>
 static hipError_t sweep_tnd(int which, const void *shots, const WorkItem *items, int nitems, int grid,
                             hipStream_t st)
 {
     const Shot<T, C> *sh = static_cast<const Shot<T, C> *>(shots);
     const dim3 block(TX, TY / NY), g(grid);
-    if (which == 0) hipLaunchKernelGGL((k_velocity_wl<T, C, TY, NY, DM>), g, block, 0, st, sh, items, nitems);
-    else hipLaunchKernelGGL((k_stress_wl<T, C, TY, NY, DM>), g, block, 0, st, sh, items, nitems);
+    if (which == 0) hipLaunchKernelGGL((k_velocity_wl<T, C, TY, NY, DM, CP>), g, block, 0, st, sh, items, nitems);
+    else hipLaunchKernelGGL((k_stress_wl<T, C, TY, NY, DM, CP>), g, block, 0, st, sh, items, nitems);
     return hipGetLastError();
 }
 
-template <class T, class C, int TY, int NY>
-static hipError_t sweep_tn(int which, int dm, const void *shots, const WorkItem *items, int nitems, int grid,
-                           hipStream_t st)
+template <class T, class C, int TY, int NY, bool CP>
+static hipError_t sweep_tnc(int which, int dm, const void *shots, const WorkItem *items, int nitems, int grid,
+                            hipStream_t st)
 {
     switch (dm) {
-    case 0: return sweep_tnd<T, C, TY, NY, 0>(which, shots, items, nitems, grid, st);
-    case 1: return sweep_tnd<T, C, TY, NY, 1>(which, shots, items, nitems, grid, st);
-    case 2: return sweep_tnd<T, C, TY, NY, 2>(which, shots, items, nitems, grid, st);
-    case 4: return sweep_tnd<T, C, TY, NY, 4>(which, shots, items, nitems, grid, st);
-    case 7: return sweep_tnd<T, C, TY, NY, 7>(which, shots, items, nitems, grid, st);
+    case 0: return sweep_tnd<T, C, TY, NY, 0, CP>(which, shots, items, nitems, grid, st);
+    case 1: return sweep_tnd<T, C, TY, NY, 1, CP>(which, shots, items, nitems, grid, st);
+    case 2: return sweep_tnd<T, C, TY, NY, 2, CP>(which, shots, items, nitems, grid, st);
+    case 4: return sweep_tnd<T, C, TY, NY, 4, CP>(which, shots, items, nitems, grid, st);
+    case 7: return sweep_tnd<T, C, TY, NY, 7, CP>(which, shots, items, nitems, grid, st);
     default: return hipErrorInvalidValue;
     }
 }
 
+// compensated instantiations exist for 16-bit storage only
+template <class T, class C, int TY, int NY>
+static hipError_t sweep_tn(int which, int dm, bool comp, const void *shots, const WorkItem *items, int nitems,
+                           int grid, hipStream_t st)
+{
+    if constexpr (sizeof(T) == 2) {
+        if (comp) return sweep_tnc<T, C, TY, NY, true>(which, dm, shots, items, nitems, grid, st);
+    } else {
+        if (comp) return hipErrorInvalidValue;
+    }
+    return sweep_tnc<T, C, TY, NY, false>(which, dm, shots, items, nitems, grid, st);
+}
+
 // supported (tile rows, rows per thread): (4,1) (8,1) (8,2) (16,1) (16,2);
-// damping masks 0, 1, 2, 4, 7
+// damping masks 0, 1, 2, 4, 7; comp: compensated time update (16-bit only)
 template <class T, class C>
-hipError_t sweep_t(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
-                   int grid, hipStream_t st)
+hipError_t sweep_t(int which, int ty, int ny, int dm, bool comp, const void *shots, const WorkItem *items,
+                   int nitems, int grid, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
     if (grid <= 0 || grid > nitems) grid = nitems;
-    if (ty == 4 && ny == 1) return sweep_tn<T, C, 4, 1>(which, dm, shots, items, nitems, grid, st);
-    if (ty == 8 && ny == 1) return sweep_tn<T, C, 8, 1>(which, dm, shots, items, nitems, grid, st);
-    if (ty == 8 && ny == 2) return sweep_tn<T, C, 8, 2>(which, dm, shots, items, nitems, grid, st);
-    if (ty == 16 && ny == 1) return sweep_tn<T, C, 16, 1>(which, dm, shots, items, nitems, grid, st);
-    if (ty == 16 && ny == 2) return sweep_tn<T, C, 16, 2>(which, dm, shots, items, nitems, grid, st);
+    if (ty == 4 && ny == 1) return sweep_tn<T, C, 4, 1>(which, dm, comp, shots, items, nitems, grid, st);
+    if (ty == 8 && ny == 1) return sweep_tn<T, C, 8, 1>(which, dm, comp, shots, items, nitems, grid, st);
+    if (ty == 8 && ny == 2) return sweep_tn<T, C, 8, 2>(which, dm, comp, shots, items, nitems, grid, st);
+    if (ty == 16 && ny == 1) return sweep_tn<T, C, 16, 1>(which, dm, comp, shots, items, nitems, grid, st);
+    if (ty == 16 && ny == 2) return sweep_tn<T, C, 16, 2>(which, dm, comp, shots, items, nitems, grid, st);
     return hipErrorInvalidValue;
 }
 

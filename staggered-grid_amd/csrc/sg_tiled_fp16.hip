@@ -3,10 +3,10 @@
 
 namespace sg {
 
-hipError_t sweep_fp16(int which, int ty, int ny, int dm, const void *shots, const WorkItem *items, int nitems,
+hipError_t sweep_fp16(int which, int ty, int ny, int dm, int comp, const void *shots, const WorkItem *items, int nitems,
                      int grid, hipStream_t st)
 {
-    return sweep_t<_Float16, float>(which, ty, ny, dm, shots, items, nitems, grid, st);
+    return sweep_t<_Float16, float>(which, ty, ny, dm, comp != 0, shots, items, nitems, grid, st);
 }
 
 }  // namespace sg
