@@ -150,6 +150,7 @@ def main():
                     help="interior box rows and rows per thread, PML box rows and rows per thread, "
                          "z planes per box (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--streams", type=int, default=None, help="streams per sweep (1, 2, 3)")
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
     ap.add_argument("--compare-sequential", action="store_true",
                     help="batched32: also time the shots one solver at a time")
@@ -178,6 +179,8 @@ def main():
     s = wl.solver(precision=args.precision, kernel=args.kernel, time_length=args.warmup + args.steps)
     if args.tile:
         s.set_tile(*args.tile)
+    if args.streams:
+        s.set_streams(args.streams)
     log(f"bench: warmup {args.warmup} steps")
     s.step(args.warmup)
     s.synchronize()

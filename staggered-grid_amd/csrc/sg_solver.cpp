@@ -93,6 +93,11 @@ struct sg_solver {
     int prec = 0;
     int device = 0;
     hipStream_t stream = nullptr;
+    // side streams: the PML work-lists of a sweep run beside the interior
+    // list (forked from and joined back to `stream` with events)
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+    int nstreams = 3;
     Geom g{};
     Tile tile{4, 1, 4, 1, 0};
 
@@ -273,11 +278,27 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
         SG_HIP(launch_stress_simple(s->prec, s->shot, s->stream));
     } else {
         const Tile &t = s->tile;
+        // lists of one sweep write disjoint points and splits and only read
+        // what the other sweep writes, so they may run concurrently:
+        // interior on the solver stream, x faces on side 0, the rest on side 1
+        const int ns = s->nstreams;
         for (int which = 0; which < 2; ++which) {
             if (which == 1 && e1) SG_HIP(hipEventRecord(e1, s->stream));
-            for (int l = 0; l < NLIST; ++l)
-                SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], s->comp_mask != 0, s->d_shot,
-                                    s->d_wl + s->wl_off[l], s->wl_off[l + 1] - s->wl_off[l], 0, s->stream));
+            if (ns > 1) {
+                SG_HIP(hipEventRecord(s->fork, s->stream));
+                for (int q = 0; q < ns - 1; ++q) SG_HIP(hipStreamWaitEvent(s->side[q], s->fork, 0));
+            }
+            for (int l = 0; l < NLIST; ++l) {
+                hipStream_t st = ns == 1 || l == 0 ? s->stream : (ns == 2 || l == 1 ? s->side[0] : s->side[1]);
+                SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], s->comp_mask != 0,
+                                    s->d_shot, s->d_wl + s->wl_off[l], s->wl_off[l + 1] - s->wl_off[l], 0, st));
+            }
+            if (ns > 1) {
+                for (int q = 0; q < ns - 1; ++q) {
+                    SG_HIP(hipEventRecord(s->join[q], s->side[q]));
+                    SG_HIP(hipStreamWaitEvent(s->stream, s->join[q], 0));
+                }
+            }
         }
     }
     if (e2) SG_HIP(hipEventRecord(e2, s->stream));
@@ -326,6 +347,11 @@ void free_all(sg_solver *s)
     }
     for (int m = 0; m < 3; ++m)
         if (s->model[m]) (void)hipFree(s->model[m]);
+    for (int q = 0; q < 2; ++q) {
+        if (s->side[q]) (void)hipStreamDestroy(s->side[q]);
+        if (s->join[q]) (void)hipEventDestroy(s->join[q]);
+    }
+    if (s->fork) (void)hipEventDestroy(s->fork);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -405,6 +431,11 @@ int sg_create(const sg_config *cfg, sg_solver **out)
 
     SG_TRY(hipGetDevice(&s->device));
     SG_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (int q = 0; q < 2; ++q) {
+        SG_TRY(hipStreamCreateWithFlags(&s->side[q], hipStreamNonBlocking));
+        SG_TRY(hipEventCreateWithFlags(&s->join[q], hipEventDisableTiming));
+    }
+    SG_TRY(hipEventCreateWithFlags(&s->fork, hipEventDisableTiming));
     const size_t sT = storage_size(s->prec), sC = compute_size(s->prec);
     for (int f = 0; f < NFIELD; ++f) {
         SG_TRY(hipMalloc(&s->fields[f], N * sT));
@@ -805,6 +836,17 @@ int sg_set_compensation(sg_solver *s, unsigned field_mask)
     s->comp_mask = field_mask;
     s->graph_valid = false;
     return refresh_shot(s);
+}
+
+int sg_set_streams(sg_solver *s, int nstreams)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (nstreams < 1 || nstreams > 3) return fail(SG_ERR_INVALID, "sg_set_streams: nstreams must be 1, 2 or 3");
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    s->nstreams = nstreams;
+    s->graph_valid = false;
+    return SG_OK;
 }
 
 int sg_set_rounding(sg_solver *s, int mode)

@@ -87,6 +87,7 @@ API = {
     "sg_energy": (ctypes.c_int, [_VP, _DP]),
     "sg_field_maxabs": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "sg_set_rounding": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_set_streams": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_set_compensation": (ctypes.c_int, [_VP, ctypes.c_uint]),
     "sg_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP]),
     "sg_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -327,6 +328,10 @@ class Solver:
         out = np.zeros(2, np.float64)
         _check(lib().sg_energy(self._h, out.ctypes.data_as(_DP)))
         return out
+
+    def set_streams(self, n):
+        """Streams the work-lists of a sweep are spread over (1, 2 or 3)."""
+        _check(lib().sg_set_streams(self._h, int(n)))
 
     def set_rounding(self, mode):
         """16-bit storage rounding: "nearest" (even) or "stochastic"."""
