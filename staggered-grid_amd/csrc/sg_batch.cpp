@@ -35,6 +35,10 @@ struct sg_batch {
     std::vector<sg_solver *> shots;
     int device = 0;
     hipStream_t stream = nullptr;
+    // PML work-lists run beside the interior list: x faces on side[0], the
+    // other PML lists on side[1] (fork/join with events)
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
     void *d_shots = nullptr;
     AuxDesc *d_aux = nullptr;
     WorkItem *d_items = nullptr;
@@ -80,12 +84,20 @@ int enqueue(sg_batch *b)
     const int n = (int)b->shots.size();
     SG_HIP(launch_inject_batch(b->d_shots, b->d_aux, n, b->stream));
     const Tile &t = b->tile;
-    for (int which = 0; which < 2; ++which)
+    for (int which = 0; which < 2; ++which) {
+        SG_HIP(hipEventRecord(b->fork, b->stream));
+        for (int q = 0; q < 2; ++q) SG_HIP(hipStreamWaitEvent(b->side[q], b->fork, 0));
         for (int l = 0; l < NLIST; ++l) {
             const int cnt = b->off[l + 1] - b->off[l];
+            hipStream_t st = l == 0 ? b->stream : (l == 1 ? b->side[0] : b->side[1]);
             SG_HIP(launch_sweep(which, SG_FP32, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], 0, b->d_shots,
-                                b->d_items + b->off[l], cnt, b->grid > 0 ? std::min(b->grid, cnt) : 0, b->stream));
+                                b->d_items + b->off[l], cnt, b->grid > 0 ? std::min(b->grid, cnt) : 0, st));
         }
+        for (int q = 0; q < 2; ++q) {
+            SG_HIP(hipEventRecord(b->join[q], b->side[q]));
+            SG_HIP(hipStreamWaitEvent(b->stream, b->join[q], 0));
+        }
+    }
     SG_HIP(launch_gather_batch(b->d_shots, b->d_aux, n, b->max_gather, b->stream));
     return SG_OK;
 }
@@ -96,6 +108,11 @@ void free_batch(sg_batch *b)
     if (b->d_shots) (void)hipFree(b->d_shots);
     if (b->d_aux) (void)hipFree(b->d_aux);
     if (b->d_items) (void)hipFree(b->d_items);
+    for (int q = 0; q < 2; ++q) {
+        if (b->side[q]) (void)hipStreamDestroy(b->side[q]);
+        if (b->join[q]) (void)hipEventDestroy(b->join[q]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -133,6 +150,11 @@ int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out)
         b->max_gather = std::max(b->max_gather, aux[i].nst * aux[i].ncomp);
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int q = 0; q < 2 && e == hipSuccess; ++q) {
+        e = hipStreamCreateWithFlags(&b->side[q], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&b->d_shots, sh.size());
     if (e == hipSuccess) e = hipMemcpy(b->d_shots, sh.data(), sh.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&b->d_aux, n * sizeof(AuxDesc));

@@ -95,8 +95,8 @@ struct sg_solver {
     hipStream_t stream = nullptr;
     // side streams: the PML work-lists of a sweep run beside the interior
     // list (forked from and joined back to `stream` with events)
-    hipStream_t side[2] = {nullptr, nullptr};
-    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
     int nstreams = 3;
     Geom g{};
     Tile tile{4, 1, 4, 1, 0};
@@ -289,7 +289,11 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
                 for (int q = 0; q < ns - 1; ++q) SG_HIP(hipStreamWaitEvent(s->side[q], s->fork, 0));
             }
             for (int l = 0; l < NLIST; ++l) {
-                hipStream_t st = ns == 1 || l == 0 ? s->stream : (ns == 2 || l == 1 ? s->side[0] : s->side[1]);
+                // list -> stream: 1 stream: all on the solver stream; 2: PML lists on side 0;
+                // 3: x faces on side 0, the other PML lists on side 1; 4: edges on side 2
+                static const int map[5][NLIST] = {{0}, {0, 0, 0, 0, 0}, {0, 1, 1, 1, 1}, {0, 1, 2, 2, 2}, {0, 1, 2, 2, 3}};
+                const int q = map[ns][l];
+                hipStream_t st = q == 0 ? s->stream : s->side[q - 1];
                 SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], s->comp_mask != 0,
                                     s->d_shot, s->d_wl + s->wl_off[l], s->wl_off[l + 1] - s->wl_off[l], 0, st));
             }
@@ -347,7 +351,7 @@ void free_all(sg_solver *s)
     }
     for (int m = 0; m < 3; ++m)
         if (s->model[m]) (void)hipFree(s->model[m]);
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 3; ++q) {
         if (s->side[q]) (void)hipStreamDestroy(s->side[q]);
         if (s->join[q]) (void)hipEventDestroy(s->join[q]);
     }
@@ -431,7 +435,7 @@ int sg_create(const sg_config *cfg, sg_solver **out)
 
     SG_TRY(hipGetDevice(&s->device));
     SG_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 3; ++q) {
         SG_TRY(hipStreamCreateWithFlags(&s->side[q], hipStreamNonBlocking));
         SG_TRY(hipEventCreateWithFlags(&s->join[q], hipEventDisableTiming));
     }
@@ -841,7 +845,7 @@ int sg_set_compensation(sg_solver *s, unsigned field_mask)
 int sg_set_streams(sg_solver *s, int nstreams)
 {
     if (int rc = check_solver(s)) return rc;
-    if (nstreams < 1 || nstreams > 3) return fail(SG_ERR_INVALID, "sg_set_streams: nstreams must be 1, 2 or 3");
+    if (nstreams < 1 || nstreams > 4) return fail(SG_ERR_INVALID, "sg_set_streams: nstreams must be 1 to 4");
     SG_HIP(hipSetDevice(s->device));
     SG_HIP(hipStreamSynchronize(s->stream));
     s->nstreams = nstreams;

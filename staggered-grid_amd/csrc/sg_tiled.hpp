@@ -384,6 +384,29 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     };
     fetch(k0);
 
+    // old velocities of plane kk (+ compensation residuals)
+    auto load_rmw = [&](int kk, C (&out)[3][NY]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T *p = plane(s.f[VX + c], kk, sz);
+#pragma unroll
+            for (int r = 0; r < NY; ++r) out[c][r] = Cvt<T, C>::ld(bld_nt(p, t.ro[r]), s.finv[VX + c]);
+        }
+        if (CP) {
+            // compensated update: the old value is stored value + residual
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (s.comp[VX + c]) {
+                    const T *pc = plane(s.comp[VX + c], kk, sz);
+#pragma unroll
+                    for (int r = 0; r < NY; ++r) out[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[VX + c]);
+                }
+        }
+    };
+    constexpr bool RP = DM == 0;  // interior boxes prefetch the RMW operands
+    C vpre[3][NY];
+    if (RP) load_rmw(k0, vpre);
+
     // damping: x per lane (edge tiles), y per box (y cuts at the PML planes),
     // z per plane, each only if the box's mask DM allows that axis; the
     // split groups of one component are `sstr` bytes apart
@@ -441,25 +464,20 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         if (ty == 0) L.b[TY][tx] = b_y;
         __syncthreads();
 
-        // ---- read half of the read-modify-write of plane k (issued first, so
-        // waiting for it does not wait for the prefetch below)
+        // ---- read half of the read-modify-write of plane k: loaded one plane
+        // ahead in interior boxes (RP); else issued first, so waiting for it
+        // does not wait for the prefetch below
         T *pv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) pv[c] = plane(s.f[VX + c], k, sz);
         C vold[3][NY];
+        if (RP) {
 #pragma unroll
-        for (int r = 0; r < NY; ++r)
+            for (int r = 0; r < NY; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) vold[c][r] = Cvt<T, C>::ld(bld_nt(pv[c], t.ro[r]), s.finv[VX + c]);
-        if (CP) {
-            // compensated update: the old value is stored value + residual
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (s.comp[VX + c]) {
-                    const T *pc = plane(s.comp[VX + c], k, sz);
-#pragma unroll
-                    for (int r = 0; r < NY; ++r) vold[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[VX + c]);
-                }
+                for (int c = 0; c < 3; ++c) vold[c][r] = vpre[c][r];
+        } else {
+            load_rmw(k, vold);
         }
         bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1, so[NY][3][3];
@@ -495,8 +513,11 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
             }
         }
 
-        // ---- loads for later: plane k+1 staging, queue heads
-        if (k + 1 < k1) fetch(k + 1);
+        // ---- loads for later: plane k+1 staging and RMW operands, queue heads
+        if (k + 1 < k1) {
+            fetch(k + 1);
+            if (RP) load_rmw(k + 1, vpre);
+        }
         C nxz[NY], nyz[NY], nzz[NY], nb[NY];
         {
             const bool z5 = k + 5 < g.nz, z6 = k + 6 < g.nz, z2 = k + 2 < g.nz;
@@ -687,6 +708,31 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     };
     fetch(k0);
 
+    // old stresses and lambda of plane kk (+ compensation residuals)
+    auto load_rmw = [&](int kk, C (&out)[6][NY], C (&lo)[NY]) {
+        const T *pl = plane(Flam, kk, sz);
+#pragma unroll
+        for (int r = 0; r < NY; ++r) lo[r] = Cvt<T, C>::ld(bld_nt(pl, t.ro[r]), il);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const T *p = plane(s.f[SXX + c], kk, sz);
+#pragma unroll
+            for (int r = 0; r < NY; ++r) out[c][r] = Cvt<T, C>::ld(bld_nt(p, t.ro[r]), s.finv[SXX + c]);
+        }
+        if (CP) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (s.comp[SXX + c]) {
+                    const T *pc = plane(s.comp[SXX + c], kk, sz);
+#pragma unroll
+                    for (int r = 0; r < NY; ++r) out[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[SXX + c]);
+                }
+        }
+    };
+    constexpr bool RP = DM == 0;  // interior boxes prefetch the RMW operands
+    C spre[6][NY], lpre[NY];
+    if (RP) load_rmw(k0, spre, lpre);
+
     // damping as in the velocity sweep; split groups (XX, YY, ZZ: x,y,z;
     // XY: x,y; XZ: x,z; YZ: y,z) are contiguous, `sstr` bytes apart
     constexpr bool PML = DM != 0, DX = (DM & 1) != 0, DY = (DM & 2) != 0, DZ = (DM & 4) != 0;
@@ -736,27 +782,22 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         }
         __syncthreads();
 
-        // ---- read half of the read-modify-write of plane k, issued first
+        // ---- read half of the read-modify-write of plane k: loaded one plane
+        // ahead in interior boxes (RP), else issued first
         C lam[NY];
-        const T *pl = plane(Flam, k, sz);
         T *ps[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) ps[c] = plane(s.f[SXX + c], k, sz);
         C old[6][NY];
+        if (RP) {
 #pragma unroll
-        for (int r = 0; r < NY; ++r) {
-            lam[r] = Cvt<T, C>::ld(bld_nt(pl, t.ro[r]), il);
+            for (int r = 0; r < NY; ++r) {
+                lam[r] = lpre[r];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) old[c][r] = Cvt<T, C>::ld(bld_nt(ps[c], t.ro[r]), s.finv[SXX + c]);
-        }
-        if (CP) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
-                if (s.comp[SXX + c]) {
-                    const T *pc = plane(s.comp[SXX + c], k, sz);
-#pragma unroll
-                    for (int r = 0; r < NY; ++r) old[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[SXX + c]);
-                }
+                for (int c = 0; c < 6; ++c) old[c][r] = spre[c][r];
+            }
+        } else {
+            load_rmw(k, old, lam);
         }
         if (k == src_k && s.src_saved) {
             // TRANSIENT source: the stress update starts from sigma before
@@ -820,6 +861,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         const bool more = k + 1 < k1;
         if (more) {
             fetch(k + 1);
+            if (RP) load_rmw(k + 1, spre, lpre);
             fetch_mu(k + 2);  // staged into the ring after this plane
         }
         C nvx[NY], nvy[NY], nvz[NY];

@@ -61,6 +61,26 @@ def test_tile_shapes_are_bit_identical(sgfd_mod):
         assert np.array_equal(tr, tr0), tile
 
 
+def test_stream_counts_are_bit_identical(sgfd_mod):
+    wl = _small_workload(sgfd_mod, 5, 150, 77, 61)
+    _say("streams 1..4, 150x77x61, eager and graph")
+    ref = None
+    for n in (1, 2, 3, 4):
+        for graph in (False, True):
+            s = wl.solver()
+            try:
+                s.set_streams(n)
+                s.set_graph(graph)
+                s.step(wl.nt)
+                s.synchronize()
+                out = (s.stations(), s.fields())
+            finally:
+                s.close()
+            if ref is None:
+                ref = out
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (n, graph)
+
+
 def test_autotune_keeps_results(sgfd_mod):
     wl = _small_workload(sgfd_mod, 4, 96, 90, 70)
     _say("autotune 96x90x70")
