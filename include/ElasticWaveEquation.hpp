@@ -5,7 +5,8 @@
 // the same constructor arguments (GRID is the TOTAL extent, PML and 5-cell
 // halo included; Medium holds buoyancy/lambda/mu x-fastest; one station;
 // DifferenceCoefficient; main frequency), run(), propagate(it, s),
-// sourceFunction(it) and stationDataOutput().  GPUDim is accepted and
+// sourceFunction(it), stationDataOutput(), snapShootSlice(it) and
+// snapShootVolum(it) (same files and text format).  GPUDim is accepted and
 // ignored: the library picks its own launch shape.  The default source mode
 // is the CUDA reference's (accumulating); pass SG_SOURCE_TRANSIENT for the
 // CPU reference's behaviour.  Errors throw sg::Error instead of exiting.
@@ -109,14 +110,75 @@ public:
         sg::check(sg_step(s_, 1));
     }
 
-    // all steps, then the station buffer to the host (..._concise/ElasticWaveEquation.cu:140-185)
-    void run()
+    // all steps, slices through the source every snapEvery steps from step
+    // snapFrom on (the reference: every 100 from 500), then the station
+    // buffer to the host (..._concise/ElasticWaveEquation.cu:140-185).
+    // snapEvery = 0: no snapshots.
+    void run(int snapEvery = 100, int snapFrom = 500)
     {
-        sg::check(sg_run(s_));
+        if (snapEvery <= 0) {
+            sg::check(sg_run(s_));
+        } else {
+            for (int it = 0; it < timeLength; ++it) {
+                sg::check(sg_step(s_, 1));
+                if (it % snapEvery == 0 && it >= snapFrom) snapShootSlice(it);
+            }
+        }
         long long n = 0;
         sg::check(sg_get_stations(s_, nullptr, 0, &n));
         stationDataCPU.resize((size_t)n);
         sg::check(sg_get_stations(s_, stationDataCPU.data(), n, &n));
+    }
+
+    // vx, sxx and sxy on the three grid planes through the source, one value
+    // per line, files velocityXSliceXX_YoZ_<it>.txt ... stressXYSliceXX_XoY_<it>.txt
+    // (..._concise/ElasticWaveEquation.cu:929-1005)
+    void snapShootSlice(int it, const std::string &dir = ".") const
+    {
+        const int nx = grid.xLength, ny = grid.yLength, nz = grid.zLength;
+        const int sx = sourceLocationCPU.x, sy = sourceLocationCPU.y, sz = sourceLocationCPU.z;
+        const int fields[3] = {SG_VX, SG_SXX, SG_SXY};
+        const char *names[3] = {"velocityX", "stressXX", "stressXY"};
+        std::vector<float> f((size_t)nx * ny * nz);
+        for (int q = 0; q < 3; ++q) {
+            sg::check(sg_get_field(s_, fields[q], f.data()));
+            auto at = [&](int i, int j, int k) { return f[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)]; };
+            std::ofstream yoz(dir + "/" + names[q] + "SliceXX_YoZ_" + std::to_string(it) + ".txt");
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j) yoz << at(sx, j, k) << '\n';
+            std::ofstream xoz(dir + "/" + names[q] + "SliceXX_XoZ_" + std::to_string(it) + ".txt");
+            for (int k = 0; k < nz; ++k)
+                for (int i = 0; i < nx; ++i) xoz << at(i, sy, k) << '\n';
+            std::ofstream xoy(dir + "/" + names[q] + "SliceXX_XoY_" + std::to_string(it) + ".txt");
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) xoy << at(i, j, sz) << '\n';
+        }
+    }
+
+    // sxx over the updated volume [5, n-5)^3, one value per line, file
+    // snapshot_0_<it>.txt (..._concise/ElasticWaveEquation.cu:1007-1037)
+    void snapShootVolum(int it, const std::string &dir = ".") const
+    {
+        const int nx = grid.xLength, ny = grid.yLength, nz = grid.zLength;
+        std::vector<float> f((size_t)nx * ny * nz);
+        sg::check(sg_get_field(s_, SG_SXX, f.data()));
+        std::ofstream out(dir + "/snapshot_0_" + std::to_string(it) + ".txt");
+        for (int k = 5; k < nz - 5; ++k)
+            for (int j = 5; j < ny - 5; ++j)
+                for (int i = 5; i < nx - 5; ++i) out << f[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] << '\n';
+    }
+
+    // binary variant (no counterpart in the reference): one field over the
+    // whole grid as raw little-endian float32, x fastest, file
+    // <field>_<it>.bin with field one of vx vy vz sxx syy szz sxy sxz syz
+    void snapShootBinary(int it, int field, const std::string &dir = ".") const
+    {
+        static const char *names[9] = {"vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz"};
+        if (field < 0 || field > 8) throw sg::Error(SG_ERR_RANGE, "snapShootBinary: bad field");
+        std::vector<float> f((size_t)grid.xLength * grid.yLength * grid.zLength);
+        sg::check(sg_get_field(s_, field, f.data()));
+        std::ofstream out(dir + "/" + names[field] + "_" + std::to_string(it) + ".bin", std::ios::binary);
+        out.write(reinterpret_cast<const char *>(f.data()), (std::streamsize)(f.size() * sizeof(float)));
     }
 
     // one value per line (..._concise/ElasticWaveEquation.cu:1040-1050)

@@ -456,11 +456,48 @@ class ElasticWaveEquation:
         self.solver.set_source_sample(timeIndex, sourceValue)
         self.solver.step(1)
 
-    def run(self):
-        """All timeLength steps, then the station buffer to host (ElasticWaveEquation.cu:140-185)."""
-        self.solver.run()
+    def run(self, snap_every=100, snap_from=500, snap_dir="."):
+        """All timeLength steps, slices through the source every `snap_every`
+        steps from `snap_from` on (the reference: every 100 from 500; 0 = no
+        snapshots), then the station buffer to host
+        (ElasticWaveEquation.cu:140-185)."""
+        if snap_every <= 0:
+            self.solver.run()
+        else:
+            for it in range(self.solver.steps_done, self.timeLength):
+                self.solver.step(1)
+                if it % snap_every == 0 and it >= snap_from:
+                    self.snapShootSlice(it, snap_dir)
         self.stationDataCPU = self.solver.stations()
         return self.stationDataCPU
+
+    @staticmethod
+    def _write_lines(path, values):
+        with open(path, "w") as f:
+            f.write("".join("%g\n" % float(v) for v in np.asarray(values, np.float32).ravel()))
+
+    def snapShootSlice(self, it, snap_dir="."):
+        """vx, sxx and sxy on the three grid planes through the source, one
+        value per line, in the reference's files and order
+        (ElasticWaveEquation.cu:929-1005)."""
+        sx, sy, sz = self.sourceLocationCPU
+        for name, fld in (("velocityX", "vx"), ("stressXX", "sxx"), ("stressXY", "sxy")):
+            F = self.solver.field(fld)  # (nz, ny, nx)
+            self._write_lines(os.path.join(snap_dir, f"{name}SliceXX_YoZ_{it}.txt"), F[:, :, sx])
+            self._write_lines(os.path.join(snap_dir, f"{name}SliceXX_XoZ_{it}.txt"), F[:, sy, :])
+            self._write_lines(os.path.join(snap_dir, f"{name}SliceXX_XoY_{it}.txt"), F[sz, :, :])
+
+    def snapShootVolum(self, it, snap_dir="."):
+        """sxx over the updated volume [5, n-5)^3, one value per line
+        (ElasticWaveEquation.cu:1007-1037)."""
+        F = self.solver.field("sxx")
+        self._write_lines(os.path.join(snap_dir, f"snapshot_0_{it}.txt"), F[5:-5, 5:-5, 5:-5])
+
+    def snapShootBinary(self, it, field, snap_dir="."):
+        """One field over the whole grid as raw little-endian float32 (x
+        fastest), file <field>_<it>.bin; no counterpart in the reference."""
+        name = field if isinstance(field, str) else FIELDS[field]
+        self.solver.field(name).astype("<f4").tofile(os.path.join(snap_dir, f"{name}_{it}.bin"))
 
     def stationDataOutput(self, path="stationData.txt"):
         """One value per line per recorded sample (ElasticWaveEquation.cu:1040-1050)."""
