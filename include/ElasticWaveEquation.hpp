@@ -95,6 +95,22 @@ public:
     ElasticWaveEquation &operator=(const ElasticWaveEquation &) = delete;
     ~ElasticWaveEquation() { sg_destroy(s_); }
 
+    // the CUDA reference's damping: tau 4.3 and the local S velocity
+    // sqrt(mu*b) of each cell (..._concise/ElasticMacro.h:21,26-27,
+    // ElasticWaveEquation.cu:285-288); the default is the CPU reference's
+    // per-axis profile
+    void setCudaDamping(double tauCuda = 4.3)
+    {
+        std::vector<double> g[3];
+        const int n3[3] = {grid.xLength, grid.yLength, grid.zLength};
+        const float h3[3] = {delta.x, delta.y, delta.z};
+        for (int a = 0; a < 3; ++a) {
+            g[a].resize((size_t)n3[a]);
+            sg::check(sg_damping_profile(n3[a], pml, h3[a], SG_DAMP_CUDA, 1.0, 1e-3, tauCuda, g[a].data()));
+        }
+        sg::check(sg_set_damping_local(s_, g[0].data(), g[1].data(), g[2].data()));
+    }
+
     // Ricker of the reference (..._concise/ElasticWaveEquation.cu:187-192)
     float sourceFunction(int timeIndex)
     {

@@ -84,7 +84,9 @@ typedef enum sg_rounding { SG_ROUND_NEAREST = 0, SG_ROUND_STOCHASTIC = 1 } sg_ro
 /* Damping profile formulas for sg_damping_profile(). */
 typedef enum sg_damping_kind {
     SG_DAMP_REFERENCE = 0, /* log10(1/R)*tau*V/(P*h)*(dist/P)^2  (CPUPML/ElasticMacro.h:21-23) */
-    SG_DAMP_QUADRATIC = 1  /* -3*V*ln(R)/(2*P*h)*(dist/P)^2 */
+    SG_DAMP_QUADRATIC = 1, /* -3*V*ln(R)/(2*P*h)*(dist/P)^2 */
+    SG_DAMP_CUDA = 2       /* tau*V/h*(8/15 - 0.03*P + P^2/1500)*(dist/P)^2
+                              (3DGPUwithPML_1DArraySingleGPU_concise/ElasticMacro.h:26-27) */
 } sg_damping_kind;
 
 /* Ricker delay conventions for sg_ricker(). */
@@ -124,6 +126,15 @@ int sg_set_medium(sg_solver *s, const float *buoyancy, const float *lambda, cons
  * the PML slabs are ignored).  The update factors (1 -/+ dt*d/2) are formed
  * in double on the host. */
 int sg_set_damping(sg_solver *s, const double *d_x, const double *d_y, const double *d_z);
+/* Per-cell damping of the CUDA reference path
+ * (3DGPUwithPML_1DArraySingleGPU_concise/ElasticWaveEquation.cu:285-288 and
+ * ElasticMacro.h:21,26-27): d_a(i,j,k) = g_a(index on axis a) * sqrt(mu*b)
+ * of the cell, i.e. the damping formula evaluated with the local S velocity.
+ * g_a comes from sg_damping_profile(..., v = 1, ...) (the CUDA path:
+ * SG_DAMP_REFERENCE with tau 4.3).  Uses the medium set by sg_set_medium
+ * (and follows later sg_set_medium calls); sg_set_damping switches back to
+ * per-axis damping. */
+int sg_set_damping_local(sg_solver *s, const double *g_x, const double *g_y, const double *g_z);
 
 /* Point source at absolute index (x, y, z) (SOURCELOCATION) with wavelet
  * samples s(it), it < n (steps beyond n inject nothing).  The injected

@@ -42,6 +42,8 @@ extern "C" int sg_damping_profile(int n, int pml, double h, int kind, double v, 
         d0 = std::log10(1.0 / r) * tau * v / (pml * h);  // ElasticMacro.h:21
     else if (kind == SG_DAMP_QUADRATIC)
         d0 = -3.0 * v * std::log(r) / (2.0 * pml * h);
+    else if (kind == SG_DAMP_CUDA)  // ..._concise/ElasticMacro.h:26
+        d0 = tau * v / h * (8.0 / 15.0 - 0.03 * pml + pml * (double)pml / 1500.0);
     else
         return sg::fail(SG_ERR_INVALID, "sg_damping_profile: unknown kind");
     for (int i = 0; i < n; ++i) {

@@ -107,6 +107,10 @@ struct Shot {
     // rounding residual of the last store, stored as residual * cscale
     T *comp[NFIELD];
     float cscale[NFIELD], cinv[NFIELD];
+    // per-cell damping (the CUDA reference's form, d = g(axis index) *
+    // sqrt(mu*b) of the cell): the local S velocity per split slot, or null;
+    // when set, dA holds dt*g/2 per axis and A, B are formed per point
+    const C *vs;
 };
 
 // residual scale of compensation arrays relative to the field scale: the
@@ -180,6 +184,8 @@ hipError_t launch_gather(int prec, const void *shot, const long long *idx, const
                          int ncomp, int interval, long long capacity, void *out, const int *d_step,
                          hipStream_t st);
 hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st);
+// local S velocity sqrt(mu*b) at every split slot (per-cell damping)
+hipError_t launch_fill_vs(int prec, const void *shot, void *vs, hipStream_t st);
 // largest |value| per wavefield (float bits, atomicMax into out9[9], which
 // the caller zeroes)
 hipError_t launch_maxabs(int prec, const void *shot, unsigned *out9, hipStream_t st);

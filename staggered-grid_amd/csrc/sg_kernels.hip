@@ -64,6 +64,20 @@ __device__ __forceinline__ C dbwd_g(const Shot<T, C> &s, int f, long long o, lon
     return r;
 }
 
+// per-cell damping: A, B from dt*g/2 (in A) and the slot's local S velocity
+template <class T, class C, int NA>
+__device__ __forceinline__ void local_damping(const Shot<T, C> &s, long long slot, C (&A)[NA], C (&B)[NA])
+{
+    if (!s.vs) return;
+    const C v = s.vs[slot];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const C hd = A[a] * v;
+        B[a] = (C)1 / ((C)1 + hd);
+        A[a] = ((C)1 - hd) * B[a];
+    }
+}
+
 // store of an updated field value (stochastic rounding when enabled for
 // 16-bit storage, keyed by point, step and field)
 template <class T, class C>
@@ -115,8 +129,9 @@ __global__ __launch_bounds__(256) void k_velocity_simple(const Shot<T, C> s)
     }
     const long long slot = split_slot(g, i, j, k);
     const bool dm[3] = {px, py, pz};
-    const C A[3] = {s.dA[0][i], s.dA[1][j], s.dA[2][k]};
-    const C B[3] = {s.dB[0][i], s.dB[1][j], s.dB[2][k]};
+    C A[3] = {s.dA[0][i], s.dA[1][j], s.dA[2][k]};
+    C B[3] = {s.dB[0][i], s.dB[1][j], s.dB[2][k]};
+    local_damping(s, slot, A, B);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const C inc[3] = {db[c] * d[c][0], db[c] * d[c][1], db[c] * d[c][2]};
@@ -178,8 +193,9 @@ __global__ __launch_bounds__(256) void k_stress_simple(const Shot<T, C> s)
     } else {
         const long long slot = split_slot(g, i, j, k);
         const bool dm[3] = {px, py, pz};
-        const C A[3] = {s.dA[0][i], s.dA[1][j], s.dA[2][k]};
-        const C B[3] = {s.dB[0][i], s.dB[1][j], s.dB[2][k]};
+        C A[3] = {s.dA[0][i], s.dA[1][j], s.dA[2][k]};
+        C B[3] = {s.dB[0][i], s.dB[1][j], s.dB[2][k]};
+        local_damping(s, slot, A, B);
         {
             const C inc[3] = {dl2m * dX, dl * dY, dl * dZ};
             C *const sp[3] = {s.ss[XX_X], s.ss[XX_Y], s.ss[XX_Z]};
@@ -315,6 +331,25 @@ __global__ void k_gather_batch(const Shot<float, float> *shots, const AuxDesc *a
 // ---------------------------------------------------------------------------
 // FP64 energy over the updated region
 // ---------------------------------------------------------------------------
+// sqrt(mu*b) of every damped updated point into its split slot
+// (..._concise/ElasticWaveEquation.cu:285: Vs_wave = sqrt(mu * buoyancy))
+template <class T, class C>
+__global__ __launch_bounds__(256) void k_fill_vs(const Shot<T, C> s, C *vs)
+{
+    const Geom &g = s.g;
+    const long long n = (long long)g.ux * g.uy * g.uz;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % g.ux) + HALO;
+        const long long r = t / g.ux;
+        const int j = (int)(r % g.uy) + HALO;
+        const int k = (int)(r / g.uy) + HALO;
+        if (!(in_pml(i, g.nx, g.pml) || in_pml(j, g.ny, g.pml) || in_pml(k, g.nz, g.pml))) continue;
+        const long long o = i + j * g.sy + k * g.sz;
+        const C m = Cvt<T, C>::ld(s.mu[o], 1.0f / s.mscale), b = Cvt<T, C>::ld(s.b[o], 1.0f / s.bscale);
+        vs[split_slot(g, i, j, k)] = sqrt(m * b);
+    }
+}
+
 // Largest |value| of each of the nine wavefields (unscaled), as float bits
 // (non-negative floats order like unsigned integers).
 template <class T, class C>
@@ -516,6 +551,19 @@ static hipError_t energy_t(const void *shot, double *out2, hipStream_t st)
 hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st)
 {
     SG_DISPATCH(prec, energy_t, shot, out2, st)
+}
+
+template <class T, class C>
+static hipError_t fill_vs_t(const void *shot, void *vs, hipStream_t st)
+{
+    const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
+    hipLaunchKernelGGL((k_fill_vs<T, C>), dim3(2048), dim3(256), 0, st, s, static_cast<C *>(vs));
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_vs(int prec, const void *shot, void *vs, hipStream_t st)
+{
+    SG_DISPATCH(prec, fill_vs_t, shot, vs, st)
 }
 
 template <class T, class C>

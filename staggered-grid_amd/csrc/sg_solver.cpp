@@ -124,6 +124,8 @@ struct sg_solver {
     unsigned comp_mask = 0;
     long long steps_done = 0;
     std::vector<double> damp[3];
+    bool damp_local = false;  // damp[] holds g of d = g * sqrt(mu*b) (sg_set_damping_local)
+    void *vs = nullptr;       // local S velocity per split slot (compute type)
 
     bool use_graph = false;
     bool graph_valid = false;
@@ -185,6 +187,7 @@ void fill_shot(sg_solver *s)
     h.mscale = s->mscale;
     h.round_mode = s->round_mode;
     h.d_step = s->d_step;
+    h.vs = s->damp_local ? static_cast<const C *>(s->vs) : nullptr;
     for (int f = 0; f < NFIELD; ++f) {
         h.comp[f] = static_cast<T *>(s->comp[f]);
         h.cscale[f] = s->fscale[f] * COMP_GAIN;
@@ -252,8 +255,14 @@ int upload_tables(sg_solver *s)
         std::vector<double> A(n3[a]), B(n3[a]);
         for (int i = 0; i < n3[a]; ++i) {
             const double hd = 0.5 * dt * s->damp[a][i];
-            A[i] = (1.0 - hd) / (1.0 + hd);
-            B[i] = 1.0 / (1.0 + hd);
+            if (s->damp_local) {
+                // per-cell damping: the kernels form A, B from dt*g/2 and the local velocity
+                A[i] = hd;
+                B[i] = 0.0;
+            } else {
+                A[i] = (1.0 - hd) / (1.0 + hd);
+                B[i] = 1.0 / (1.0 + hd);
+            }
         }
         buf.insert(buf.end(), A.begin(), A.end());
         buf.insert(buf.end(), B.begin(), B.end());
@@ -341,7 +350,7 @@ void free_all(sg_solver *s)
     drop_graph(s);
     for (auto ev : s->events) (void)hipEventDestroy(ev);
     s->events.clear();
-    void *ptrs[] = {s->splits, s->tables, s->saved, s->d_step, s->amp_d, s->amp_acc,
+    void *ptrs[] = {s->splits, s->tables, s->saved, s->d_step, s->amp_d, s->amp_acc, s->vs,
                     s->st_idx, s->st_comp, s->st_out, s->d_energy, s->d_shot, s->d_wl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -486,6 +495,8 @@ int sg_destroy(sg_solver *s)
     return SG_OK;
 }
 
+static int fill_vs(sg_solver *s);
+
 int sg_set_medium(sg_solver *s, const float *b, const float *lam, const float *mu)
 {
     if (int rc = check_solver(s)) return rc;
@@ -521,7 +532,7 @@ int sg_set_medium(sg_solver *s, const float *b, const float *lam, const float *m
         }
         if (int rc = refresh_shot(s)) return rc;
     }
-    return SG_OK;
+    return fill_vs(s);
 }
 
 int sg_set_field_scales(sg_solver *s, const float scales[9])
@@ -564,7 +575,40 @@ int sg_set_damping(sg_solver *s, const double *dx, const double *dy, const doubl
     SG_HIP(hipSetDevice(s->device));
     SG_HIP(hipStreamSynchronize(s->stream));
     for (int a = 0; a < 3; ++a) s->damp[a].assign(d3[a], d3[a] + n3[a]);
-    return upload_tables(s);
+    const bool was_local = s->damp_local;
+    s->damp_local = false;
+    if (int rc = upload_tables(s)) return rc;
+    return was_local ? refresh_shot(s) : SG_OK;
+}
+
+// the local S velocity of every split slot from the medium on the device
+static int fill_vs(sg_solver *s)
+{
+    if (!s->damp_local || s->g.nslots == 0) return SG_OK;
+    if (!s->vs) SG_HIP(hipMalloc(&s->vs, (size_t)s->g.nslots * compute_size(s->prec)));
+    if (int rc = refresh_shot(s)) return rc;
+    SG_HIP(launch_fill_vs(s->prec, s->shot, s->vs, s->stream));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    return SG_OK;
+}
+
+int sg_set_damping_local(sg_solver *s, const double *gx, const double *gy, const double *gz)
+{
+    if (int rc = check_solver(s)) return rc;
+    const double *g3[3] = {gx, gy, gz};
+    const int n3[3] = {s->g.nx, s->g.ny, s->g.nz};
+    for (int a = 0; a < 3; ++a) {
+        if (!g3[a]) return fail(SG_ERR_INVALID, "sg_set_damping_local: NULL profile");
+        for (int i = 0; i < n3[a]; ++i)
+            if (!std::isfinite(g3[a][i]) || g3[a][i] < 0)
+                return fail(SG_ERR_INVALID, "sg_set_damping_local: g must be finite and >= 0");
+    }
+    SG_HIP(hipSetDevice(s->device));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    for (int a = 0; a < 3; ++a) s->damp[a].assign(g3[a], g3[a] + n3[a]);
+    s->damp_local = true;
+    if (int rc = upload_tables(s)) return rc;
+    return fill_vs(s);
 }
 
 int sg_set_source(sg_solver *s, int x, int y, int z, const float *wavelet, int n)

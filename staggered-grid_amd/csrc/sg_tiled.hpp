@@ -196,6 +196,20 @@ __device__ __forceinline__ C mld(const T *__restrict__ p, unsigned boff, bool ok
     return ok ? v : (C)0;
 }
 
+// Per-cell damping (the CUDA reference's form): A holds dt*g/2 per axis on
+// entry; with the point's local S velocity v, hd = A*v and
+// A <- (1-hd)/(1+hd), B <- 1/(1+hd).
+template <class C>
+__device__ __forceinline__ void vs_damping(C v, C (&A)[3], C (&B)[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const C hd = A[a] * v;
+        B[a] = (C)1 / ((C)1 + hd);
+        A[a] = ((C)1 - hd) * B[a];
+    }
+}
+
 // Split-field PML update of one component from the old splits `so` of its
 // contributing axes (loaded by the caller): damped axes f_a <- A_a f_a +
 // B_a inc_a (3DCPUwithPML.../ElasticWaveEquation3D.cpp:153-158), new splits
@@ -480,7 +494,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
             load_rmw(k, vold);
         }
         bool pz = false, any[NY];
-        C A2 = (C)1, B2 = (C)1, so[NY][3][3];
+        C A2 = (C)1, B2 = (C)1, so[NY][3][3], vsr[NY];
         unsigned sb[NY];
         if (PML) {
             if (DZ) {
@@ -496,6 +510,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int a2 = 0; a2 < 3; ++a2) so[r][c][a2] = (C)0;
+                vsr[r] = (s.vs && any[r]) ? bld(s.vs, sb[r]) : (C)0;
                 if (any[r]) {
                     if (DX && px) {
 #pragma unroll
@@ -560,8 +575,9 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 // split stores come after the field stores
                 const bool ar = any[r];
                 const bool dm[3] = {ar && px, ar && py, ar && pz};
-                const C A[3] = {A0, A1[r], A2};
-                const C B[3] = {B0, B1[r], B2};
+                C A[3] = {A0, A1[r], A2};
+                C B[3] = {B0, B1[r], B2};
+                if (s.vs) vs_damping(vsr[r], A, B);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const C inc[3] = {db[c] * d[c][0], db[c] * d[c][1], db[c] * d[c][2]};
@@ -815,6 +831,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1;
         C sn3[NY][3][3], s2[NY][3][2];  // old splits: normal [comp][x,y,z]; shear xy, xz, yz
+        C vsr[NY];
         unsigned sb[NY];
         if (PML) {
             if (DZ) {
@@ -833,6 +850,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #pragma unroll
                     for (int a2 = 0; a2 < 2; ++a2) s2[r][c][a2] = (C)0;
                 }
+                vsr[r] = (s.vs && any[r]) ? bld(s.vs, sb[r]) : (C)0;
                 if (any[r]) {
                     const unsigned o = sb[r];
                     if (DX && px) {
@@ -912,8 +930,9 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 const bool ar = any[r];
                 const bool mx = ar && px, my = ar && py, mz = ar && pz;
                 const bool dm[3] = {mx, my, mz};
-                const C A[3] = {A0, A1[r], A2};
-                const C B[3] = {B0, B1[r], B2};
+                C A[3] = {A0, A1[r], A2};
+                C B[3] = {B0, B1[r], B2};
+                if (s.vs) vs_damping(vsr[r], A, B);
                 const C dnorm[3] = {dX, dY, dZ};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {

@@ -25,7 +25,7 @@ FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 PRECISIONS = {"fp32": 0, "fp64": 1, "fp16": 2, "bf16": 3}
 SOURCE_MODES = {"transient": 0, "accumulate": 1}
 KERNELS = {"tiled": 0, "simple": 1}
-DAMP_REFERENCE, DAMP_QUADRATIC = 0, 1
+DAMP_REFERENCE, DAMP_QUADRATIC, DAMP_CUDA = 0, 1, 2
 RICKER_CEIL_DELAY, RICKER_NO_DELAY = 0, 1
 
 
@@ -87,6 +87,7 @@ API = {
     "sg_energy": (ctypes.c_int, [_VP, _DP]),
     "sg_field_maxabs": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "sg_set_rounding": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_set_damping_local": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "sg_set_streams": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_set_compensation": (ctypes.c_int, [_VP, ctypes.c_uint]),
     "sg_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP]),
@@ -151,7 +152,7 @@ def ricker(f0: float, dt: float, nt: int, delay: str = "ceil") -> np.ndarray:
 
 def damping_profile(n, pml, h, kind="reference", v=2000.0, r=1e-3, tau=3.4) -> np.ndarray:
     out = np.zeros(n, np.float64)
-    k = DAMP_REFERENCE if kind == "reference" else DAMP_QUADRATIC
+    k = {"reference": DAMP_REFERENCE, "quadratic": DAMP_QUADRATIC, "cuda": DAMP_CUDA}[kind]
     _check(lib().sg_damping_profile(n, pml, h, k, v, r, tau, out.ctypes.data_as(_DP)))
     return out
 
@@ -227,6 +228,13 @@ class Solver:
         arrs = [np.ascontiguousarray(a, np.float64) for a in (dx, dy, dz)]
         assert arrs[0].size == self.nx and arrs[1].size == self.ny and arrs[2].size == self.nz
         _check(lib().sg_set_damping(self._h, *[a.ctypes.data_as(_DP) for a in arrs]))
+
+    def set_damping_local(self, gx, gy, gz):
+        """Per-cell damping d = g(axis index) * sqrt(mu*b) of the cell (the
+        CUDA reference path's form); g from damping_profile(..., v=1)."""
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (gx, gy, gz)]
+        assert arrs[0].size == self.nx and arrs[1].size == self.ny and arrs[2].size == self.nz
+        _check(lib().sg_set_damping_local(self._h, *[a.ctypes.data_as(_DP) for a in arrs]))
 
     def set_source(self, x, y, z, wavelet):
         w = np.ascontiguousarray(wavelet, np.float32)
@@ -421,7 +429,9 @@ class ElasticWaveEquation:
     ``gpuDim`` is accepted and ignored (launch shapes are chosen by the
     library).  Defaults follow the reference CUDA path: sxx recorded,
     accumulating source, reference damping formula; pass
-    ``source_mode="transient"`` for the CPU path's behaviour.
+    ``source_mode="transient"`` for the CPU path's behaviour and
+    ``damping="cuda"`` for the CUDA path's per-cell damping (tau 4.3, local
+    S velocity; ..._concise/ElasticMacro.h:21,26-27).
     """
 
     def __init__(self, timeLength, gpuDim, pml, grid, delta, medCPU, sourceLocation, stationLocation, dc,
@@ -438,7 +448,12 @@ class ElasticWaveEquation:
                              self.delta.x, self.delta.y, self.delta.z, coef=dc, time_length=timeLength,
                              precision=precision, source_mode=source_mode, kernel=kernel)
         self.solver.set_medium(*medCPU)
-        if damping is not None:
+        if isinstance(damping, str) and damping == "cuda":
+            self.solver.set_damping_local(*[damping_profile(n, pml, h, "cuda", v=1.0, tau=4.3)
+                                           for n, h in ((self.grid.xLength, self.delta.x),
+                                                        (self.grid.yLength, self.delta.y),
+                                                        (self.grid.zLength, self.delta.z))])
+        elif damping is not None:
             self.solver.set_damping(*damping)
         self._wavelet = np.array([self.sourceFunction(it) for it in range(timeLength)], np.float32)
         self.solver.set_source(*self.sourceLocationCPU, self._wavelet)
