@@ -1,6 +1,7 @@
 // The reference CUDA driver program (3DGPUwithPML_1DArraySingleGPU_concise/main.cu)
 // ported to the drop-in header: same grid, medium, source, station and
-// coefficients; writes stationData.txt.  Argument 1 (optional) = time steps.
+// coefficients and damping; writes stationData.txt and the reference's slice
+// snapshots.  Argument 1 (optional) = time steps.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -22,7 +23,8 @@ int main(int argc, char **argv)
     try {
         ElasticWaveEquation ewq(timeLength, GPUDim{}, pml, grid, delta, medCPU, sourceLocation, stationLocation, dc,
                                 20.0f);
-        ewq.run();
+        ewq.setCudaDamping();  // the CUDA reference's damping (tau 4.3, local S velocity)
+        ewq.run();             // slices through the source every 100 steps from 500, as the reference
         ewq.stationDataOutput();
         std::printf("recorded %zu samples, sample[235] = %g\n", ewq.stationDataCPU.size(),
                     ewq.stationDataCPU.size() > 235 ? ewq.stationDataCPU[235] : 0.0f);

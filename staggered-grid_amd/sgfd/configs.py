@@ -217,4 +217,24 @@ def batched_shots(seed=77, nshots=32, nt=800):
     return shots
 
 
-BUILDERS = {"plain100": plain100, "random256": random256, "layered512": layered512, "large1024": large1024}
+def nopml500(nt=1000) -> Workload:
+    """The configuration of the reference's published GPU timing
+    (3DGPUwithoutPML_1DArraySingleGPU/main.cu:11-41): 500^3 with no PML
+    (5-cell halo only), dx 5 m, dt 0.5 ms, 1000 steps, homogeneous rho 2500,
+    lambda = rho 2000^2, mu = lambda/3, Ricker 30 Hz at the centre."""
+    n = 500
+    shape = (n, n, n)
+    rho = np.float32(2500.0)
+    lam = np.float32(2500.0 * 2000.0 ** 2)
+    b = np.full(shape, np.float32(1.0) / rho, np.float32)
+    lam_a = np.full(shape, lam, np.float32)
+    mu_a = np.full(shape, np.float32(lam / np.float32(3.0)), np.float32)
+    vp = float(np.sqrt((lam + 2 * lam / 3) / rho))
+    st = np.array([[n // 2 + 20, n // 2, n // 2], [n // 2, n // 2 + 20, n // 2]], np.int32)
+    return Workload("nopml500", n, n, n, 0, 5e-4, 5.0, 5.0, 5.0, nt, b, lam_a, mu_a, (n // 2, n // 2, n // 2),
+                    ricker(30.0, 5e-4, nt, "none"), st, damp_v=vp, vp_max=vp,
+                    description="500^3 no PML, 1000 steps (the reference's published GPU timing case)", f0=30.0)
+
+
+BUILDERS = {"plain100": plain100, "random256": random256, "layered512": layered512, "large1024": large1024,
+            "nopml500": nopml500}
