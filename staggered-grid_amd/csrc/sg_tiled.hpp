@@ -1033,13 +1033,16 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #ifndef SG_PML_WAVES
 #define SG_PML_WAVES 1
 #endif
-template <int DM>
+#ifndef SG_INT_WAVES
+#define SG_INT_WAVES 1
+#endif
+template <int DM, int NY>
 struct WavesCap {
-    static constexpr int v = DM ? SG_PML_WAVES : 1;
+    static constexpr int v = DM ? SG_PML_WAVES : (NY == 1 ? SG_INT_WAVES : 1);
 };
 
 template <class T, class C, int TY, int NY, int DM, bool CP>
-__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM>::v))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM, NY>::v))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
                                                             const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ VelLds<C, TY> L;
@@ -1051,7 +1054,7 @@ __global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(Wav
 }
 
 template <class T, class C, int TY, int NY, int DM, bool CP>
-__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM>::v))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<DM, NY>::v))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
                                                           const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ StrLds<C, TY> L;
