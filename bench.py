@@ -138,6 +138,54 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
+def bench_medium(args):
+    """The seeded random-medium builder (the reference's spectral method,
+    randomMedium/RandomMedia_3_d.m, with its own spectrum and parameters) on
+    a 512x512x384 grid: value = grid points / device time of one build (the
+    FFT plan and the copy of the result to the host are outside the timing)."""
+    import sgfd
+
+    nx, ny, nz = 512, 512, 384
+    n = nx * ny * nz
+    K = min(args.steps, 20)
+    for w in range(max(args.warmup, 1)):
+        sgfd.random_medium_timed(nx, ny, nz, "reference", seed=w)
+    ms = []
+    for k in range(K):
+        _, t = sgfd.random_medium_timed(nx, ny, nz, "reference", seed=100 + k)
+        ms.append(t)
+    avg = sum(ms) / K
+    # spectrum write 8 B, transform read + write 16 B (one-pass lower bound),
+    # two moment passes 16 B, scaling read 8 B + write 4 B
+    alg = 52.0 * n
+    out = {"metric": "random-medium build rate", "value": n / (avg * 1e-3) / 1e6, "unit": "Mpt/s", "n_gpus": 1,
+           "steps": K, "warmup": max(args.warmup, 1), "ms_per_step": avg, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 moments)", "data": "synthetic",
+           "config": {"workload": "random medium 512x512x384, the reference script's spectrum (a=10, p=10, l=50), "
+                                  "v0 2000, eps 0.1", "grid": [nx, ny, nz], "parallelism": "single-gpu"},
+           "roofline": {"bound": "hbm", "kernel": "whole build (spectrum, hipFFT C2C, moments, scaling)",
+                        "achieved": alg / (avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS, "traffic": None,
+                        "alg_bytes_per_launch": alg, "avg_launch_ms": avg}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import medium as M
+
+        log("bench: CPU baseline (numpy oracle, 128x128x96)")
+        cx, cy, cz = 128, 128, 96
+        t = time.perf_counter()
+        reps = 0
+        while reps < 3 or time.perf_counter() - t < 10.0:
+            M.random_medium(cx, cy, cz, "reference", seed=reps)
+            reps += 1
+        sec = (time.perf_counter() - t) / reps
+        out["cpu_baseline"] = {"value": cx * cy * cz / sec / 1e6, "unit": "Mpt/s", "cores": 1, "kind": "port",
+                               "sample": f"numpy restatement, {cx}x{cy}x{cz}, {reps} builds"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +218,9 @@ def main():
         return
     if args.workload == "batched32":
         bench_batched(args)
+        return
+    if args.workload == "medium512":
+        bench_medium(args)
         return
 
     log(f"bench: building workload {args.workload}")

@@ -274,6 +274,54 @@ int sg_diff_coef(int kind, float out[5]);
  * a = corr cells), seeded and reproducible, clipped to +-clip; x-fastest. */
 int sg_gaussian_field(int nx, int ny, int nz, double corr, unsigned long long seed, double clip, float *out);
 
+/* Raw float32 model files of the reference's MATLAB tools (V.dat,
+ * randomMedium/RandomMedia_3_d.m:101-103, fwrite(Fid, V, 'float')): MATLAB
+ * writes V(i,j,k) in column order, i fastest, which is this library's
+ * x-fastest layout with x = i.  Reading requires the file to hold exactly n
+ * floats. */
+int sg_read_raw_f32(const char *path, long long n, float *out);
+int sg_write_raw_f32(const char *path, long long n, const float *in);
+
+/* ---- random media on the GPU (spectral method) ------------------------------ */
+
+/* Spectrum of sg_random_medium, indexed by the signed bin index s of each
+ * axis (s = m for m <= (n-1)/2, else m - n):
+ *   REFERENCE:  (a0 a1 a2) / (1 + (kx^2 a0^2)^p + (ky^2 a1^2)^p + (kz^2 a2^2)^p),
+ *               k = s / l, p = order: the form of randomMedium/RandomMedia_3_d.m:29
+ *               (a = b = c = 10, p = 10, l = 50 there); the field is |psi|
+ *               as in the script (:66);
+ *   GAUSSIAN:   exp(-(kx^2 a0^2 + ky^2 a1^2 + kz^2 a2^2) / 4), k = 2 pi s / n
+ *               (radians per cell, a in cells): correlation exp(-(x/a0)^2 - ...);
+ *   VON_KARMAN: (1 + kx^2 a0^2 + ky^2 a1^2 + kz^2 a2^2)^-(order + 3/2)
+ *               (order 0.5: exponential correlation);
+ * the Gaussian and von Karman fields are Re psi. */
+typedef enum sg_spectrum_kind {
+    SG_SPECTRUM_REFERENCE = 0,
+    SG_SPECTRUM_GAUSSIAN = 1,
+    SG_SPECTRUM_VON_KARMAN = 2
+} sg_spectrum_kind;
+
+typedef struct sg_spectrum {
+    int kind;     /* sg_spectrum_kind */
+    double a[3];  /* correlation lengths along x, y, z */
+    double order; /* REFERENCE: exponent p; VON_KARMAN: order (> 0) */
+    double l;     /* REFERENCE: wavenumber scale */
+} sg_spectrum;
+
+/* Seeded random medium by the reference's spectral method
+ * (randomMedium/RandomMedia_3_d.m:20-93) on the current HIP device:
+ * psi = ifft3(sqrt(P) e^{i theta}) with theta uniform from a counter-based
+ * generator keyed by seed and bin, the field's mean removed, scaled to
+ * standard deviation eps: out = v0 (1 + delta), or delta itself when
+ * v0 <= 0.  nz x ny x nx floats, x fastest, into host memory. */
+int sg_random_medium(int nx, int ny, int nz, const sg_spectrum *spec, unsigned long long seed, double v0, double eps,
+                     float *out);
+/* The same, with the device time of the build (HIP events around the
+ * spectrum, transform, moments and scaling; the FFT plan and the copy of the
+ * result to the host excluded) in milliseconds. */
+int sg_random_medium_timed(int nx, int ny, int nz, const sg_spectrum *spec, unsigned long long seed, double v0,
+                           double eps, float *out, double *device_ms);
+
 /* Last error message of the calling thread ("" if none). */
 const char *sg_last_error(void);
 int sg_api_version(void);

@@ -168,3 +168,38 @@ extern "C" int sg_gaussian_field(int nx, int ny, int nz, double corr, unsigned l
     });
     return SG_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Raw float32 model files (V.dat of randomMedium/RandomMedia_3_d.m:101-103):
+// n native little-endian floats, no header.
+// ---------------------------------------------------------------------------
+#include <cstdio>
+
+extern "C" int sg_read_raw_f32(const char *path, long long n, float *out)
+{
+    if (!path || !out || n < 1) return sg::fail(SG_ERR_INVALID, "sg_read_raw_f32: invalid arguments");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return sg::fail(SG_ERR_INVALID, std::string("sg_read_raw_f32: cannot open ") + path);
+    long long size = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) size = (long long)std::ftell(f);
+    if (size != n * (long long)sizeof(float)) {
+        std::fclose(f);
+        return sg::fail(SG_ERR_INVALID, std::string("sg_read_raw_f32: ") + path + " holds " + std::to_string(size) +
+                                            " bytes, expected " + std::to_string(n * (long long)sizeof(float)));
+    }
+    std::rewind(f);
+    const size_t got = std::fread(out, sizeof(float), (size_t)n, f);
+    std::fclose(f);
+    if (got != (size_t)n) return sg::fail(SG_ERR_INVALID, std::string("sg_read_raw_f32: short read of ") + path);
+    return SG_OK;
+}
+
+extern "C" int sg_write_raw_f32(const char *path, long long n, const float *in)
+{
+    if (!path || !in || n < 1) return sg::fail(SG_ERR_INVALID, "sg_write_raw_f32: invalid arguments");
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return sg::fail(SG_ERR_INVALID, std::string("sg_write_raw_f32: cannot create ") + path);
+    const size_t put = std::fwrite(in, sizeof(float), (size_t)n, f);
+    const bool ok = std::fclose(f) == 0 && put == (size_t)n;
+    return ok ? SG_OK : sg::fail(SG_ERR_INVALID, std::string("sg_write_raw_f32: write to ") + path + " failed");
+}

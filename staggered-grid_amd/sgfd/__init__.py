@@ -58,6 +58,14 @@ class _Tile(ctypes.Structure):
 
 
 _TP = ctypes.POINTER(_Tile)
+
+
+class _Spectrum(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("a", ctypes.c_double * 3), ("order", ctypes.c_double),
+                ("l", ctypes.c_double)]
+
+
+SPECTRA = {"reference": 0, "gaussian": 1, "von_karman": 2}
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
 _LLP = ctypes.POINTER(ctypes.c_longlong)
@@ -109,6 +117,12 @@ API = {
     "sg_diff_coef": (ctypes.c_int, [ctypes.c_int, _FP]),
     "sg_gaussian_field": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_ulonglong, ctypes.c_double, _FP]),
+    "sg_read_raw_f32": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_longlong, _FP]),
+    "sg_write_raw_f32": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_longlong, _FP]),
+    "sg_random_medium": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Spectrum),
+                                        ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double, _FP]),
+    "sg_random_medium_timed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Spectrum),
+                                              ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double, _FP, _DP]),
 }
 
 
@@ -162,6 +176,46 @@ def gaussian_field(nx, ny, nz, corr, seed, clip=4.0) -> np.ndarray:
     out = np.zeros((nz, ny, nx), np.float32)
     _check(lib().sg_gaussian_field(nx, ny, nz, float(corr), int(seed), float(clip), _fptr(out)))
     return out
+
+
+def read_vdat(path, nx, ny, nz) -> np.ndarray:
+    """Raw float32 model file as written by the reference's MATLAB tools
+    (randomMedium/RandomMedia_3_d.m:101-103: fwrite of V(i,j,k), i fastest),
+    returned as an (nz, ny, nx) array with x = i."""
+    out = np.empty((nz, ny, nx), np.float32)
+    _check(lib().sg_read_raw_f32(os.fsencode(path), out.size, _fptr(out)))
+    return out
+
+
+def write_vdat(path, v) -> None:
+    """Write an (nz, ny, nx) array as raw float32, x fastest (MATLAB's
+    fread(fid, [nx*ny*nz], 'float') order for V(i,j,k))."""
+    a = np.ascontiguousarray(v, np.float32)
+    _check(lib().sg_write_raw_f32(os.fsencode(path), a.size, _fptr(a)))
+
+
+def random_medium(nx, ny, nz, kind="reference", a=(10.0, 10.0, 10.0), order=10.0, l=50.0, seed=0,
+                  v0=2000.0, eps=0.1) -> np.ndarray:
+    """Seeded random medium by the reference's spectral method on the GPU
+    (randomMedium/RandomMedia_3_d.m:20-93): v0 * (1 + delta) with delta of
+    zero mean and standard deviation eps (delta itself when v0 <= 0), shape
+    (nz, ny, nx).  kind: "reference" (the script's spectrum; its defaults are
+    a = 10, order = 10, l = 50, v0 = 2000, eps = 0.1), "gaussian" or
+    "von_karman" (a in cells; order = von Karman order)."""
+    out, _ = random_medium_timed(nx, ny, nz, kind, a, order, l, seed, v0, eps)
+    return out
+
+
+def random_medium_timed(nx, ny, nz, kind="reference", a=(10.0, 10.0, 10.0), order=10.0, l=50.0, seed=0,
+                        v0=2000.0, eps=0.1):
+    """random_medium plus the device time of the build in ms (FFT plan and
+    the copy to the host excluded)."""
+    sp = _Spectrum(SPECTRA[kind], (ctypes.c_double * 3)(*[float(x) for x in a]), float(order), float(l))
+    out = np.empty((nz, ny, nx), np.float32)
+    ms = ctypes.c_double()
+    _check(lib().sg_random_medium_timed(nx, ny, nz, ctypes.byref(sp), int(seed), float(v0), float(eps), _fptr(out),
+                                        ctypes.byref(ms)))
+    return out, ms.value
 
 
 def copy_bandwidth(nbytes: int, reps: int = 20) -> float:
