@@ -227,8 +227,9 @@ int sg_set_tile(sg_solver *s, const sg_tile *tile);
 int sg_get_tile(const sg_solver *s, sg_tile *tile);
 /* Streams the work-lists of one sweep are spread over: 1 = one after the
  * other on the solver stream; 2 = PML lists beside the interior list;
- * 3 (default) = x faces and the other PML lists on two side streams.  The
- * lists write disjoint points, so results do not change. */
+ * 3 = x faces and the other PML lists on two side streams.  Default: 3, or 1
+ * for grids below 4 Mi points (launch-bound).  The lists write disjoint
+ * points, so results do not change. */
 int sg_set_streams(sg_solver *s, int nstreams);
 /* Autotuner: times `steps` steps for a set of interior shapes, PML shapes
  * and z-chunk lengths, keeps the fastest (written to *best; per-step sweep

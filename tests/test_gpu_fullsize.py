@@ -4,6 +4,9 @@
   OpenMP build finishes them in seconds): every wavefield rel-L2 <= 1e-5.
 * 512x512x384 four-layer model (P20), the full 2000 steps: FP32 station
   traces against the FP64 build of the same kernels, rel-L2 <= 1e-5.
+* 1024x1024x640 four-layer model (P24), the full 1500 steps: FP32 against
+  the FP64 build on the 256x256 surface receivers and on a seeded 2 %
+  subsample of the points of vz and sxx, rel-L2 <= 1e-5.
 * 512x512x384, size-independent property: the update is linear in the
   source amplitude and scaling by 2 is exact in binary floating point, so a
   doubled wavelet must give doubled fields.  Where the leading edge of the
@@ -78,6 +81,28 @@ def test_layered512_fp32_against_fp64_full_run(mods):
     e = _rel(res["fp32"], res["fp64"])
     _say(f"seismogram rel-L2 fp32 vs fp64: {e:.3e}")
     assert e < 1e-5
+
+
+def test_large1024_fp32_against_fp64_subsample(mods):
+    sgfd, configs = mods
+    _say("large1024: building the model")
+    wl = configs.large1024()
+    n = wl.nx * wl.ny * wl.nz
+    idx = np.random.default_rng(1024).choice(n, size=n // 50, replace=False)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        _say(f"large1024 1500 steps {prec}")
+        s = wl.solver(prec)
+        try:
+            s.step(wl.nt)
+            s.synchronize()
+            res[prec] = [s.stations().astype(np.float64)] + [s.field(f, np.float64).ravel()[idx] for f in ("vz", "sxx")]
+        finally:
+            s.close()
+    for what, a, b in zip(("stations", "vz 2 %", "sxx 2 %"), res["fp32"], res["fp64"]):
+        e = _rel(a, b)
+        _say(f"large1024 {what} rel-L2 fp32 vs fp64: {e:.3e}")
+        assert e < 1e-5, what
 
 
 def test_layered512_linearity(mods):
