@@ -67,6 +67,32 @@ def cpu_baseline(wl, planes=60, steps=4):
             "host_cpus": os.cpu_count()}
 
 
+def cpu_all_cores(wl, planes=60, steps=4):
+    """The C restatement of the reference CPU solver with its loop nest
+    split over up to 16 host threads, on the same bounded sample as
+    cpu_baseline (informative beside the one-thread reference)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    nz = min(planes, wl.nz)
+    sl = slice(0, nz)
+    b, lam, mu = (np.ascontiguousarray(a[sl]) for a in (wl.b, wl.lam, wl.mu))
+    src = (wl.src[0], wl.src[1], min(wl.src[2], nz // 2))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    p = O.OracleParams(wl.nx, wl.ny, nz, wl.pml, wl.dt, wl.dx, wl.dy, wl.dz, steps, src,
+                       coef=tuple(float(c) for c in wl.coef), damp_mode=O.DAMP_QUADRATIC, damp_v=wl.damp_v,
+                       nthreads=threads)
+    w = np.asarray(wl.wavelet[:steps], np.float32)
+    t = time.perf_counter()
+    O.run(p, b, lam, mu, w, None, fields=False)
+    el = time.perf_counter() - t
+    return {"value": wl.nx * wl.ny * nz * steps / el / 1e9, "unit": "Gpt-updates/s", "cores": threads,
+            "kind": "port", "sample": f"oracle/sg_oracle.c (gcc -O2, {threads} threads) on z-planes 0..{nz - 1} "
+                                      f"of the same model, {steps} steps, {el:.2f} s"}
+
+
 def read_traffic(workload, precision):
     """HBM bytes per stress launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, copy-kernel calibrated), if a summary is committed."""
@@ -272,8 +298,10 @@ def main():
         "energy_after": list(energy),
     }
     if not args.no_cpu:
-        log("bench: CPU baseline (oracle, bounded sample)")
+        log("bench: CPU baseline (reference CPU solver, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(wl)
+        log("bench: CPU restatement on all host cores (same sample)")
+        out["cpu_all_cores"] = cpu_all_cores(wl)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

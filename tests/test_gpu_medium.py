@@ -100,3 +100,28 @@ def test_random_medium_feeds_the_solver(sgfd_mod):
     finally:
         s.close()
     assert np.all(np.isfinite(vz)) and np.abs(vz).max() > 0
+
+
+def test_cpp_random_medium_workflow(sgfd_mod, tmp_path):
+    """examples/random_medium.cpp: the script's model built through the C ABI,
+    written as V.dat, read back and run through the drop-in class; the file
+    equals the Python call with the same seed and its statistics are v0 and
+    eps * v0."""
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib_dir = os.path.join(root, "staggered-grid_amd", "sgfd")
+    exe = tmp_path / "random_medium"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(root, "include"),
+                    os.path.join(root, "staggered-grid_amd", "examples", "random_medium.cpp"), "-L" + lib_dir,
+                    "-lsgfd", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
+    _say("C++ random-medium program")
+    r = subprocess.run([str(exe), str(tmp_path)], check=True, capture_output=True, text=True, timeout=120)
+    assert "run done" in r.stdout
+    v = sgfd_mod.read_vdat(str(tmp_path / "V.dat"), 96, 96, 96)
+    assert np.array_equal(v, sgfd_mod.random_medium(96, 96, 96, "reference", seed=7, v0=2000.0, eps=0.1))
+    v64 = v.astype(np.float64)
+    assert abs(v64.mean() - 2000.0) < 1e-3 and abs(v64.std() - 200.0) < 1e-3
+    rows = open(tmp_path / "stationData.txt").read().split()
+    assert len(rows) == 200 and np.all(np.isfinite(np.array(rows, np.float64)))
