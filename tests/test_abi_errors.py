@@ -1,0 +1,80 @@
+"""Error behaviour of the C ABI that needs no GPU: every configuration check
+of sg_create runs before any device call, and every solver entry point
+rejects a NULL handle; each failure returns its sg_status (never exits) and
+leaves a message in sg_last_error.  (The reference exits through CHECK,
+common.h:6-16.)"""
+import ctypes
+
+import pytest
+
+SG_OK, SG_ERR_INVALID = 0, -1
+
+
+@pytest.fixture(scope="module")
+def L():
+    import sgfd
+
+    sgfd.lib()
+    return sgfd.lib()
+
+
+def _cfg(**kw):
+    import sgfd
+
+    c = sgfd._Config(64, 64, 64, 8, 1e-3, 10.0, 10.0, 10.0,
+                     (ctypes.c_float * 5)(1.2, -0.09, 0.014, -0.0018, 0.00012), 100, 0, 0, 0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+@pytest.mark.parametrize("kw,needle", [
+    ({"nx": 10}, b"extent"),
+    ({"nz": 0}, b"extent"),
+    ({"pml": 28}, b"pml"),
+    ({"pml": -1}, b"pml"),
+    ({"dt": 0.0}, b"positive"),
+    ({"dy": -1.0}, b"positive"),
+    ({"time_length": -1}, b"time_length"),
+    ({"precision": 4}, b"precision"),
+    ({"source_mode": 2}, b"source_mode"),
+    ({"kernel": 2}, b"kernel"),
+    ({"nx": 1300, "ny": 1300, "nz": 1300, "pml": 0}, b"2^31"),
+])
+def test_create_rejects_bad_configurations(L, kw, needle):
+    h = ctypes.c_void_p(1)
+    assert L.sg_create(ctypes.byref(_cfg(**kw)), ctypes.byref(h)) == SG_ERR_INVALID
+    assert h.value is None  # the out handle is cleared
+    assert needle in L.sg_last_error()
+
+
+def test_create_rejects_null_arguments(L):
+    h = ctypes.c_void_p()
+    assert L.sg_create(None, ctypes.byref(h)) == SG_ERR_INVALID
+    assert L.sg_create(ctypes.byref(_cfg()), None) == SG_ERR_INVALID
+
+
+@pytest.mark.parametrize("name,args", [
+    ("sg_step", (1,)),
+    ("sg_run", ()),
+    ("sg_synchronize", ()),
+    ("sg_reset", ()),
+    ("sg_set_graph", (1,)),
+    ("sg_set_streams", (2,)),
+    ("sg_set_rounding", (0,)),
+    ("sg_set_compensation", (0,)),
+    ("sg_set_source_sample", (0, 1.0)),
+])
+def test_null_solver_is_rejected(L, name, args):
+    fn = getattr(L, name)
+    assert fn(None, *args) == SG_ERR_INVALID
+    assert b"NULL" in L.sg_last_error()
+
+
+def test_destroy_null_is_a_no_op(L):
+    assert L.sg_destroy(None) == SG_OK
+
+
+def test_batch_needs_shots(L):
+    out = ctypes.c_void_p()
+    assert L.sg_batch_create(None, 0, ctypes.byref(out)) == SG_ERR_INVALID
