@@ -45,10 +45,10 @@ def cpu_baseline(wl, planes=60, steps=4):
     b, lam, mu = (np.ascontiguousarray(a[sl]) for a in (wl.b, wl.lam, wl.mu))
     src = (wl.src[0], wl.src[1], min(wl.src[2], nz // 2))
     pts = wl.nx * wl.ny * nz
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_baseline")
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_workload")
     if os.path.exists(exe):
         P = wl.pml
-        el = O.ref_time("baseline", (wl.nx - 2 * P - 10, wl.ny - 2 * P - 10, nz - 2 * P - 10), P, steps, src,
+        el = O.ref_time("workload", (wl.nx - 2 * P - 10, wl.ny - 2 * P - 10, nz - 2 * P - 10), P, steps, src,
                         wl.dx, wl.dy, wl.dz, wl.dt, wl.f0 or 15.0, wl.damp_v, b, lam, mu)
         return {"value": pts * steps / el / 1e9, "unit": "Gpt-updates/s", "cores": 1, "kind": "reference",
                 "sample": f"reference CPU solver (ElasticWaveEquation3D.cpp, g++ -O2, 1 thread) on z-planes "
@@ -232,7 +232,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+              "(multi-GPU sharding is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import sgfd

@@ -12,7 +12,7 @@
  *    (ElasticMacro.h:140) with 5 arguments, which GCC rejects.  The expansion
  *    is unchanged (it declares nine pointers).
  *  - private members are made public so the harness can read the arrays.
- *  - With -DSG_BASELINE_CONSTANTS the three constants that differ from the
+ *  - With -DSG_WORKLOAD_CONSTANTS the three constants that differ from the
  *    stated workload are overridden: DIFF_10's coefficients (exact rationals),
  *    DAMPING0 (d0 = -3 V ln(R) / (2 P delta)) with a run-time V, and the Ricker
  *    delay (ceil() -> 0, so the peak sits at 1/f0).
@@ -49,7 +49,7 @@
     DEF_3D_POINTER(type, velocityY##name)        \
     DEF_3D_POINTER(type, velocityZ##name)
 
-#ifdef SG_BASELINE_CONSTANTS
+#ifdef SG_WORKLOAD_CONSTANTS
 static float sg_damp_v = 3000.0f;
 static const float SGC0 = (float)(19845.0 / 16384.0);
 static const float SGC1 = (float)(-735.0 / 8192.0);
@@ -95,7 +95,7 @@ int main(int argc, char **argv)
     if (nst && !rd(fi, st.data(), st.size() * sizeof(int))) return 5;
     if (!rd(fi, model.data(), model.size() * sizeof(float))) return 6;
     fclose(fi);
-#ifdef SG_BASELINE_CONSTANTS
+#ifdef SG_WORKLOAD_CONSTANTS
     sg_damp_v = fh[5];
 #endif
 

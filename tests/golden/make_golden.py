@@ -16,8 +16,9 @@ inputs and stores its outputs as small .npz fixtures next to this file.
       staggering), unequal dx/dy/dz, final fields + traces.
   G2  the 100^3 CPU-runnable workload (P=10, 500 steps): 20 stations x
       (vx,vy,vz,sxx) traces and the final fields on a stride-5 subsample.
-Also copies nothing from the reference except, separately, the data file
-stationData.txt (see README in this folder).
+Nothing else comes from the reference except one data file kept next to
+this script: stationData_concise_gpu.txt, the reference's own
+3DGPUwithPML_1DArraySingleGPU_concise/stationData.txt (a fixture).
 """
 import json
 import os
@@ -81,10 +82,10 @@ def g1():
                    [src[0], src[1], NZ - 7], [src[0], src[1], src[2]]], np.int32)
     dx, dy, dz = 10.0, 12.0, 8.0
     dt, f0, dampv = 1e-3, 15.0, 3000.0
-    R = O.ref_run("baseline", interior, pml, nt, src, st, dx, dy, dz, dt, f0, dampv, b, lam, mu)
-    meta = dict(kind="baseline", interior=interior, total=(NX, NY, NZ), pml=pml, nt=nt, src=src,
+    R = O.ref_run("workload", interior, pml, nt, src, st, dx, dy, dz, dt, f0, dampv, b, lam, mu)
+    meta = dict(kind="workload", interior=interior, total=(NX, NY, NZ), pml=pml, nt=nt, src=src,
                 dx=dx, dy=dy, dz=dz, dt=dt, f0=f0, damp_v=dampv)
-    save("g1_baseline_hetero.npz", meta, stations=st, b=b, lam=lam, mu=mu, wavelet=R["wavelet"],
+    save("g1_workload_hetero.npz", meta, stations=st, b=b, lam=lam, mu=mu, wavelet=R["wavelet"],
          traces=R["traces"], fields=R["fields"], damp_x=R["damping"][0], damp_y=R["damping"][1],
          damp_z=R["damping"][2])
 
@@ -99,19 +100,19 @@ def g2():
     src = (50, 50, 50)
     st = np.array([[20 + 3 * r, 50, 20] for r in range(20)], np.int32)
     t = time.time()
-    R = O.ref_run("baseline", interior, pml, nt, src, st, 10.0, 10.0, 10.0, 1e-3, 15.0, 3000.0, b, lam, mu)
+    R = O.ref_run("workload", interior, pml, nt, src, st, 10.0, 10.0, 10.0, 1e-3, 15.0, 3000.0, b, lam, mu)
     print("G2 reference run", time.time() - t, "s")
     sub = R["fields"][:, 2::5, 2::5, 2::5]
-    meta = dict(kind="baseline", interior=interior, total=(NX, NY, NZ), pml=pml, nt=nt, src=src,
+    meta = dict(kind="workload", interior=interior, total=(NX, NY, NZ), pml=pml, nt=nt, src=src,
                 dx=10.0, dy=10.0, dz=10.0, dt=1e-3, f0=15.0, damp_v=3000.0, vp=3000.0, vs=1732.0, rho=2000.0,
                 subsample="fields[:, 2::5, 2::5, 2::5]", components="vx,vy,vz,sxx",
                 field_sumsq=[float(np.sum(R["fields"][f].astype(np.float64) ** 2)) for f in range(9)])
-    save("g2_config1_traces.npz", meta, stations=st, wavelet=R["wavelet"], traces=R["traces"][:, :, :4],
+    save("g2_plain100_traces.npz", meta, stations=st, wavelet=R["wavelet"], traces=R["traces"][:, :, :4],
          fields_sub=sub)
 
 
 if __name__ == "__main__":
-    if not (O.ref_available("native") and O.ref_available("baseline")):
+    if not (O.ref_available("native") and O.ref_available("workload")):
         sys.exit("reference harness not built: run `make -C oracle ref` first")
     g0()
     g1()

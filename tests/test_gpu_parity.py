@@ -83,7 +83,7 @@ def _golden_case(O, name):
 
 
 @pytest.mark.parametrize("kernel", ["tiled", "simple"])
-@pytest.mark.parametrize("name", ["g0_native_homog.npz", "g1_baseline_hetero.npz"])
+@pytest.mark.parametrize("name", ["g0_native_homog.npz", "g1_workload_hetero.npz"])
 def test_golden_fp32(oracle_mod, sgfd_mod, name, kernel):
     O = oracle_mod
     meta, g, p, b, lam, mu, w = _golden_case(O, name)
@@ -101,7 +101,7 @@ def test_golden_fp32(oracle_mod, sgfd_mod, name, kernel):
 @pytest.mark.parametrize("kernel", ["tiled", "simple"])
 def test_golden_fp64_matches_fp64_oracle(oracle_mod, sgfd_mod, kernel):
     O = oracle_mod
-    meta, g, p, b, lam, mu, w = _golden_case(O, "g1_baseline_hetero.npz")
+    meta, g, p, b, lam, mu, w = _golden_case(O, "g1_workload_hetero.npz")
     _say(f"fp64 g1 kernel={kernel}")
     tr, F = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], sgfd_mod.FIELDS, kernel=kernel, precision="fp64",
                    damp=_damp_for(O, p))
@@ -113,8 +113,8 @@ def test_golden_fp64_matches_fp64_oracle(oracle_mod, sgfd_mod, kernel):
     assert _rel(tr, tr64) < 1e-6
 
 
-def _config1(O):
-    meta, g = load_golden("g2_config1_traces.npz")
+def _plain100(O):
+    meta, g = load_golden("g2_plain100_traces.npz")
     NX, NY, NZ = meta["total"]
     rho = np.full((NZ, NY, NX), meta["rho"], np.float64)
     mu = (rho * meta["vs"] ** 2)
@@ -130,7 +130,7 @@ def _config1(O):
 _ORACLE_CACHE = {}
 
 
-def _oracle_config1(O, p, b, lam, mu, w, nt, dtype):
+def _oracle_plain100(O, p, b, lam, mu, w, nt, dtype):
     key = (nt, np.dtype(dtype).name)
     if key not in _ORACLE_CACHE:
         import dataclasses
@@ -145,7 +145,7 @@ def _oracle_config1(O, p, b, lam, mu, w, nt, dtype):
 
 
 @pytest.mark.parametrize("kernel", ["tiled", "simple"])
-def test_config1_full_run(oracle_mod, sgfd_mod, kernel):
+def test_plain100_full_run(oracle_mod, sgfd_mod, kernel):
     """100^3 incl. a 10-cell PML, 500 steps.
 
     * station traces of the whole run against the compiled reference's golden
@@ -158,8 +158,8 @@ def test_config1_full_run(oracle_mod, sgfd_mod, kernel):
       must be at least as close to FP64 as the reference's (factor 1.5).
     """
     O = oracle_mod
-    meta, g, p, b, lam, mu, w = _config1(O)
-    _say(f"config1 100^3 500 steps kernel={kernel}")
+    meta, g, p, b, lam, mu, w = _plain100(O)
+    _say(f"plain100 100^3 500 steps kernel={kernel}")
     s = sgfd_mod.Solver(p.nx, p.ny, p.nz, p.pml, p.dt, p.dx, p.dy, p.dz, coef=p.coef, time_length=p.nt,
                         kernel=kernel)
     try:
@@ -179,25 +179,25 @@ def test_config1_full_run(oracle_mod, sgfd_mod, kernel):
     assert _rel(tr[:, :, :3], g["traces"][:, :, :3]) < TOL32
     for c in (0, 2, 3):
         assert _rel(tr[:, :, c], g["traces"][:, :, c]) < TOL32
-    _say("config1 oracle runs (150 steps f32, 500 steps f32/f64)")
-    O150 = _oracle_config1(O, p, b, lam, mu, w, 150, np.float32)
+    _say("plain100 oracle runs (150 steps f32, 500 steps f32/f64)")
+    O150 = _oracle_plain100(O, p, b, lam, mu, w, 150, np.float32)
     for f in range(9):
         assert _rel(F150[f], O150[f]) < TOL32, sgfd_mod.FIELDS[f]
-    O500 = _oracle_config1(O, p, b, lam, mu, w, p.nt, np.float32)
-    O500d = _oracle_config1(O, p, b, lam, mu, w, p.nt, np.float64)
+    O500 = _oracle_plain100(O, p, b, lam, mu, w, p.nt, np.float32)
+    O500d = _oracle_plain100(O, p, b, lam, mu, w, p.nt, np.float64)
     ref_err = _rel(O500, O500d)
     assert ref_err < 1e-3
     assert _rel(F, O500d) < 1.5 * ref_err
     assert _rel(F[:, 2::5, 2::5, 2::5], g["fields_sub"]) < 2 * ref_err
 
 
-def test_config1_fp64_matches_fp64_oracle(oracle_mod, sgfd_mod):
+def test_plain100_fp64_matches_fp64_oracle(oracle_mod, sgfd_mod):
     O = oracle_mod
-    meta, g, p, b, lam, mu, w = _config1(O)
-    _say("config1 fp64 tiled")
+    meta, g, p, b, lam, mu, w = _plain100(O)
+    _say("plain100 fp64 tiled")
     tr, F = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], ("vx", "vy", "vz", "sxx"), precision="fp64",
                    damp=_damp_for(O, p))
-    O500d = _oracle_config1(O, p, b, lam, mu, w, p.nt, np.float64)
+    O500d = _oracle_plain100(O, p, b, lam, mu, w, p.nt, np.float64)
     assert _rel(F, O500d) < 1e-9
 
 
@@ -279,7 +279,7 @@ def test_no_pml_against_oracle(oracle_mod, sgfd_mod, kernel):
 
 def test_graph_replay_matches_eager(oracle_mod, sgfd_mod):
     O = oracle_mod
-    meta, g, p, b, lam, mu, w = _golden_case(O, "g1_baseline_hetero.npz")
+    meta, g, p, b, lam, mu, w = _golden_case(O, "g1_workload_hetero.npz")
     _say("graph replay vs eager, g1")
     d = _damp_for(O, p)
     tr1, F1 = _solve(sgfd_mod, p, b, lam, mu, w, g["stations"], sgfd_mod.FIELDS, damp=d)

@@ -112,7 +112,7 @@ def test_g0_native_constants_bitexact(oracle_mod):
 
 def test_g1_heterogeneous_bitexact(oracle_mod):
     O = oracle_mod
-    meta, g = load_golden("g1_baseline_hetero.npz")
+    meta, g = load_golden("g1_workload_hetero.npz")
     p = _params_from_meta(O, meta, native=False)
     w = O.ricker(meta["f0"], meta["dt"], meta["nt"], O.RICKER_NO_DELAY)
     assert np.array_equal(w, g["wavelet"])
@@ -125,9 +125,9 @@ def test_g1_heterogeneous_bitexact(oracle_mod):
         assert np.array_equal(d, g[key])
 
 
-def test_g2_config1_traces_bitexact(oracle_mod):
+def test_g2_plain100_traces_bitexact(oracle_mod):
     O = oracle_mod
-    meta, g = load_golden("g2_config1_traces.npz")
+    meta, g = load_golden("g2_plain100_traces.npz")
     NX, NY, NZ = meta["total"]
     b, lam, mu = _model_homog((NZ, NY, NX), meta["vp"], meta["vs"], meta["rho"])
     p = _params_from_meta(O, meta, native=False)
@@ -221,7 +221,7 @@ def test_halo_stays_zero_and_axis_symmetry(oracle_mod):
 
 def test_fp64_oracle_agrees_with_fp32(oracle_mod):
     O = oracle_mod
-    meta, g = load_golden("g1_baseline_hetero.npz")
+    meta, g = load_golden("g1_workload_hetero.npz")
     p = _params_from_meta(O, meta, native=False)
     w = O.ricker(meta["f0"], meta["dt"], meta["nt"], O.RICKER_NO_DELAY)
     tr32, F32 = O.run(p, g["b"], g["lam"], g["mu"], w, g["stations"])
