@@ -226,9 +226,10 @@ typedef struct sg_tile {
 int sg_set_tile(sg_solver *s, const sg_tile *tile);
 int sg_get_tile(const sg_solver *s, sg_tile *tile);
 /* Streams the work-lists of one sweep are spread over: 1 = one after the
- * other on the solver stream; 2 = PML lists beside the interior list;
- * 3 = x faces and the other PML lists on two side streams.  Default: 3, or 1
- * for grids below 4 Mi points (launch-bound).  The lists write disjoint
+ * other on the solver stream (with (4,1) boxes for the interior and the PML,
+ * all five lists as one launch per sweep); 2 = PML lists beside the interior
+ * list; 3 = x faces and the other PML lists on two side streams.  Default: 3,
+ * or 1 for grids below 4 Mi points (launch-bound).  The lists write disjoint
  * points, so results do not change. */
 int sg_set_streams(sg_solver *s, int nstreams);
 /* Autotuner: times `steps` steps for a set of interior shapes, PML shapes

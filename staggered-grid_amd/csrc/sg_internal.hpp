@@ -175,6 +175,19 @@ hipError_t sweep_fp16(int which, int ty, int ny, int dm, int comp, const void *s
 hipError_t sweep_bf16(int which, int ty, int ny, int dm, int comp, const void *shots, const WorkItem *items, int nitems,
                       int grid, hipStream_t st);
 
+// all work-lists of a sweep in one launch, (4,1) boxes (off: NLIST+1 list
+// offsets into items)
+hipError_t sweep_fused_fp32(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+                            hipStream_t st);
+hipError_t sweep_fused_fp64(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+                            hipStream_t st);
+hipError_t sweep_fused_fp16(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+                            hipStream_t st);
+hipError_t sweep_fused_bf16(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+                            hipStream_t st);
+hipError_t launch_sweep_fused(int which, int prec, int comp, const void *shots_dev, const WorkItem *items,
+                              const int *off, int grid, hipStream_t st);
+
 // work-lists: interior, x face, y face, z face, edges/corners
 constexpr int NLIST = 5;
 constexpr int LIST_DM[NLIST] = {0, 1, 2, 4, 7};

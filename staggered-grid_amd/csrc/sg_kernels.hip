@@ -516,6 +516,18 @@ hipError_t launch_sweep(int which, int prec, int ty, int ny, int dm, int comp, c
     }
 }
 
+hipError_t launch_sweep_fused(int which, int prec, int comp, const void *shots_dev, const WorkItem *items,
+                              const int *off, int grid, hipStream_t st)
+{
+    switch (prec) {
+    case 0: return sweep_fused_fp32(which, comp, shots_dev, items, off, grid, st);
+    case 1: return sweep_fused_fp64(which, comp, shots_dev, items, off, grid, st);
+    case 2: return sweep_fused_fp16(which, comp, shots_dev, items, off, grid, st);
+    case 3: return sweep_fused_bf16(which, comp, shots_dev, items, off, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc, int namp,
                          int accumulate, int *d_step, void *saved, hipStream_t st)
 {

@@ -291,7 +291,14 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
         // what the other sweep writes, so they may run concurrently:
         // interior on the solver stream, x faces on side 0, the rest on side 1
         const int ns = s->nstreams;
-        for (int which = 0; which < 2; ++which) {
+        // one stream and (4,1) boxes throughout: the five lists of a sweep in
+        // one launch (small grids are launch-bound)
+        const bool fused = ns == 1 && t.ty == 4 && t.ny == 1 && t.pty == 4 && t.pny == 1;
+        for (int which = 0; fused && which < 2; ++which) {
+            if (which == 1 && e1) SG_HIP(hipEventRecord(e1, s->stream));
+            SG_HIP(launch_sweep_fused(which, s->prec, s->comp_mask != 0, s->d_shot, s->d_wl, s->wl_off, 0, s->stream));
+        }
+        for (int which = 0; !fused && which < 2; ++which) {
             if (which == 1 && e1) SG_HIP(hipEventRecord(e1, s->stream));
             if (ns > 1) {
                 SG_HIP(hipEventRecord(s->fork, s->stream));

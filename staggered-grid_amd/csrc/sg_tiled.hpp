@@ -1131,4 +1131,90 @@ hipError_t sweep_t(int which, int ty, int ny, int dm, bool comp, const void *sho
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// all five work-lists of a sweep in one launch (small, launch-bound grids):
+// each item's list, found from the list offsets, selects the damping-mask
+// instantiation; the per-item code is that of the separate launches, so
+// results are bit-identical.  (4,1) boxes for interior and PML alike.
+// ---------------------------------------------------------------------------
+struct ListOffsets {
+    int off[NLIST + 1];
+};
+
+__device__ __forceinline__ int list_of_item(const ListOffsets &lo, int p)
+{
+    int l = 0;
+#pragma unroll
+    for (int q = 1; q < NLIST; ++q) l += p >= lo.off[q] ? 1 : 0;
+    return l;
+}
+
+template <class T, class C, bool CP>
+__global__ __launch_bounds__(TX * 4) void k_velocity_fused(const Shot<T, C> *__restrict__ shots,
+                                                           const WorkItem *__restrict__ items, ListOffsets lo)
+{
+    __shared__ VelLds<C, 4> L;
+    for (int p = blockIdx.x; p < lo.off[NLIST]; p += gridDim.x) {
+        const WorkItem w = items[p];
+        if (w.shot >= 0) {
+            const Shot<T, C> &s = shots[w.shot];
+            switch (list_of_item(lo, p)) {
+            case 0: velocity_tile<T, C, 4, 1, 0, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 1: velocity_tile<T, C, 4, 1, 1, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 2: velocity_tile<T, C, 4, 1, 2, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 3: velocity_tile<T, C, 4, 1, 4, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            default: velocity_tile<T, C, 4, 1, 7, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class T, class C, bool CP>
+__global__ __launch_bounds__(TX * 4) void k_stress_fused(const Shot<T, C> *__restrict__ shots,
+                                                         const WorkItem *__restrict__ items, ListOffsets lo)
+{
+    __shared__ StrLds<C, 4> L;
+    for (int p = blockIdx.x; p < lo.off[NLIST]; p += gridDim.x) {
+        const WorkItem w = items[p];
+        if (w.shot >= 0) {
+            const Shot<T, C> &s = shots[w.shot];
+            switch (list_of_item(lo, p)) {
+            case 0: stress_tile<T, C, 4, 1, 0, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 1: stress_tile<T, C, 4, 1, 1, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 2: stress_tile<T, C, 4, 1, 2, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 3: stress_tile<T, C, 4, 1, 4, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            default: stress_tile<T, C, 4, 1, 7, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// off: list offsets into items (off[NLIST] items in all)
+template <class T, class C>
+hipError_t sweep_fused_t(int which, bool comp, const void *shots, const WorkItem *items, const int *off, int grid,
+                         hipStream_t st)
+{
+    ListOffsets lo;
+    for (int l = 0; l <= NLIST; ++l) lo.off[l] = off[l];
+    const int n = lo.off[NLIST];
+    if (n <= 0) return hipSuccess;
+    if (grid <= 0 || grid > n) grid = n;
+    const Shot<T, C> *sh = static_cast<const Shot<T, C> *>(shots);
+    const dim3 block(TX, 4), g(grid);
+    if constexpr (sizeof(T) == 2) {
+        if (comp) {
+            if (which == 0) hipLaunchKernelGGL((k_velocity_fused<T, C, true>), g, block, 0, st, sh, items, lo);
+            else hipLaunchKernelGGL((k_stress_fused<T, C, true>), g, block, 0, st, sh, items, lo);
+            return hipGetLastError();
+        }
+    } else {
+        if (comp) return hipErrorInvalidValue;
+    }
+    if (which == 0) hipLaunchKernelGGL((k_velocity_fused<T, C, false>), g, block, 0, st, sh, items, lo);
+    else hipLaunchKernelGGL((k_stress_fused<T, C, false>), g, block, 0, st, sh, items, lo);
+    return hipGetLastError();
+}
+
 }  // namespace sg
