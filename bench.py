@@ -271,6 +271,11 @@ def main():
     K = args.steps
     sec = ms_tot / 1e3
     value = wl.points * K / sec / 1e9
+    # updated points (5-cell halo excluded) and the undamped interior among them
+    upd = (wl.nx - 10) * (wl.ny - 10) * (wl.nz - 10)
+    inner = max(wl.nx - 10 - 2 * wl.pml, 0) * max(wl.ny - 10 - 2 * wl.pml, 0) * max(wl.nz - 10 - 2 * wl.pml, 0)
+    # the roofline also as a fraction of this device's measured copy bandwidth
+    copy_gbps = sgfd.copy_bandwidth(1 << 30, 20)
     avg_v, avg_s = ms_v / K, ms_s / K
     dom = ("stress", avg_s, sbytes) if avg_s >= avg_v else ("velocity", avg_v, vbytes)
     achieved = dom[2] / (dom[1] * 1e-3) / 1e9
@@ -289,10 +294,12 @@ def main():
         "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": wl.description, "grid": [wl.nx, wl.ny, wl.nz], "pml": wl.pml,
+                   "points": {"total": wl.points, "updated": upd, "interior": inner},
                    "precision": args.precision, "kernel": args.kernel, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "alg_bytes_per_launch": dom[2], "avg_launch_ms": dom[1]},
+                     "alg_bytes_per_launch": dom[2], "avg_launch_ms": dom[1],
+                     "copy_gbps": copy_gbps, "copy_frac": achieved / copy_gbps},
         "sweeps_ms": {"velocity_avg": avg_v, "stress_avg": avg_s},
         "step_hbm_frac": (vbytes + sbytes) * K / sec / 1e9 / HBM_PEAK_GBPS,
         "energy_after": list(energy),
