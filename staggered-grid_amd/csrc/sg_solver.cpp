@@ -406,7 +406,10 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     if (c.kernel != SG_KERNEL_TILED && c.kernel != SG_KERNEL_SIMPLE)
         return fail(SG_ERR_INVALID, "sg_create: bad kernel");
     const long long N = (long long)c.nx * c.ny * c.nz;
-    if (N > (1LL << 31) - 1) return fail(SG_ERR_INVALID, "sg_create: grid exceeds 2^31-1 points per array");
+    // indices are 64-bit; the tiled sweeps address inside one xy plane with
+    // 32-bit byte offsets
+    if ((long long)c.nx * c.ny * 8 >= (1LL << 32))
+        return fail(SG_ERR_INVALID, "sg_create: an xy plane must hold fewer than 2^29 points");
 
     sg_solver *s = new (std::nothrow) sg_solver();
     if (!s) return fail(SG_ERR_NOMEM, "sg_create: host allocation failed");

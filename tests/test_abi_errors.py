@@ -39,7 +39,7 @@ def _cfg(**kw):
     ({"precision": 4}, b"precision"),
     ({"source_mode": 2}, b"source_mode"),
     ({"kernel": 2}, b"kernel"),
-    ({"nx": 1300, "ny": 1300, "nz": 1300, "pml": 0}, b"2^31"),
+    ({"nx": 32768, "ny": 16384, "nz": 11, "pml": 0}, b"plane"),
 ])
 def test_create_rejects_bad_configurations(L, kw, needle):
     h = ctypes.c_void_p(1)

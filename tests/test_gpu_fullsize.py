@@ -7,6 +7,11 @@
 * 1024x1024x640 four-layer model (P24), the full 1500 steps: FP32 against
   the FP64 build on the 256x256 surface receivers and on a seeded 2 %
   subsample of the points of vz and sxx, rel-L2 <= 1e-5.
+* 1344x1344x1300 (2.35 G points, beyond 2^31 per array), homogeneous,
+  source 100 cells below the top: after 20 steps the wave has spread at most
+  200 cells, so a 440x440x330 grid with the source at the same distance from
+  its top boundary must hold the same numbers, bit for bit (64-bit indexing
+  of the large arrays, the source above linear index 2^31).
 * 512x512x384, size-independent property: the update is linear in the
   source amplitude and scaling by 2 is exact in binary floating point, so a
   doubled wavelet must give doubled fields.  Where the leading edge of the
@@ -130,3 +135,43 @@ def test_layered512_linearity(mods):
         e = _rel(2 * a, b)
         _say(f"fp32 doubled: rel-L2 {e:.2e}")
         assert e < 1e-6
+
+
+def test_grid_beyond_2p31_points_matches_small_grid(mods):
+    sgfd, _ = mods
+    pml, nt, h, dt = 10, 20, 10.0, 1e-3
+    rho, vp = 2000.0, 3000.0
+    mu_v, lam_v = rho * (vp / np.sqrt(3)) ** 2, rho * vp ** 2 - 2 * rho * (vp / np.sqrt(3)) ** 2
+    w = sgfd.ricker(15.0, dt, nt, "none")
+
+    def run(nx, ny, nz, src):
+        n = nx * ny * nz
+        s = sgfd.Solver(nx, ny, nz, pml, dt, h, h, h, coef=sgfd.diff_coef("exact"), time_length=nt)
+        try:
+            b = np.full(n, 1 / rho, np.float32)
+            lam = np.full(n, lam_v, np.float32)
+            mu = np.full(n, mu_v, np.float32)
+            s.set_medium(b, lam, mu)
+            del b, lam, mu
+            s.set_damping(*[sgfd.damping_profile(m, pml, h, "quadratic", vp) for m in (nx, ny, nz)])
+            s.set_source(*src, w)
+            s.step(nt)
+            s.synchronize()
+            return {f: s.field(f) for f in ("vz", "sxx", "sxy")}
+        finally:
+            s.close()
+
+    big_shape, big_src = (1344, 1344, 1300), (672, 672, 1200)
+    small_shape, small_src = (440, 440, 330), (220, 220, 230)
+    assert np.prod(big_shape) > 2 ** 31 and (big_src[2] * big_shape[1] + big_src[1]) * big_shape[0] > 2 ** 31
+    _say("small grid 440x440x330")
+    small = run(*small_shape, small_src)
+    _say("large grid 1344x1344x1300 (2.35 G points)")
+    big = run(*big_shape, big_src)
+    lo = [big_src[a] - small_src[a] for a in range(3)]
+    sl = (slice(lo[2], lo[2] + small_shape[2]), slice(lo[1], lo[1] + small_shape[1]), slice(lo[0], lo[0] + small_shape[0]))
+    for f in small:
+        assert np.abs(small[f]).max() > 0
+        assert np.array_equal(big[f][sl], small[f]), f
+        # nothing outside the compared block was reached
+        assert np.count_nonzero(big[f]) == np.count_nonzero(small[f]), f
