@@ -432,12 +432,11 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     g.sy = c.nx;
     g.sz = (long long)c.nx * c.ny;
     split_layout(g);
-    // the tiled sweeps address one component's splits (up to three arrays of
-    // nslots values) with 32-bit byte offsets
-    if (c.kernel == SG_KERNEL_TILED && 3 * g.nslots * (long long)compute_size(c.precision) >= (1LL << 32)) {
+    // the tiled sweeps address inside one split array with 32-bit byte offsets
+    if (c.kernel == SG_KERNEL_TILED && g.nslots * (long long)compute_size(c.precision) >= (1LL << 32)) {
         delete s;
-        return fail(SG_ERR_UNSUPPORTED, "sg_create: PML split store too large for the tiled sweeps "
-                                        "(3 * slots * element size >= 4 GiB); use SG_KERNEL_SIMPLE");
+        return fail(SG_ERR_UNSUPPORTED, "sg_create: PML split arrays too large for the tiled sweeps "
+                                        "(slots * element size >= 4 GiB); use SG_KERNEL_SIMPLE");
     }
 
     auto bail = [&](int rc) {

@@ -424,13 +424,12 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     if (RP) load_rmw(k0, vpre);
 
     // damping: x per lane (edge tiles), y per box (y cuts at the PML planes),
-    // z per plane, each only if the box's mask DM allows that axis; the
-    // split groups of one component are `sstr` bytes apart
+    // z per plane, each only if the box's mask DM allows that axis; every
+    // split array is addressed from its own base with the point's slot offset
     constexpr bool PML = DM != 0, DX = (DM & 1) != 0, DY = (DM & 2) != 0, DZ = (DM & 4) != 0;
     const bool px = DX && in_pml(t.i, g.nx, g.pml);
     const bool py = DY && in_pml(j0, g.ny, g.pml);
     C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
-    const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
     // step number keying stochastic rounding (16-bit storage only)
     const uint32_t step = (sizeof(T) == 2 && s.round_mode) ? (uint32_t)gld(s.d_step, 0u) : 0u;
     if (DX) {
@@ -520,11 +519,11 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                     }
                     if (DY && py) {
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) so[r][c][1] = bld(s.sv[3 * c], sb[r] + sstr);
+                        for (int c = 0; c < 3; ++c) so[r][c][1] = bld(s.sv[3 * c + 1], sb[r]);
                     }
                     if (DZ && pz) {
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) so[r][c][2] = bld(s.sv[3 * c], sb[r] + 2 * sstr);
+                        for (int c = 0; c < 3; ++c) so[r][c][2] = bld(s.sv[3 * c + 2], sb[r]);
                     }
                 }
             }
@@ -609,11 +608,11 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 }
                 if (DY && py) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) bst(s.sv[3 * c], sb[r] + sstr, sn[c][1]);
+                    for (int c = 0; c < 3; ++c) bst(s.sv[3 * c + 1], sb[r], sn[c][1]);
                 }
                 if (DZ && pz) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) bst(s.sv[3 * c], sb[r] + 2 * sstr, sn[c][2]);
+                    for (int c = 0; c < 3; ++c) bst(s.sv[3 * c + 2], sb[r], sn[c][2]);
                 }
             }
         }
@@ -753,13 +752,12 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     C spre[6][NY], lpre[NY];
     if (RP) load_rmw(k0, spre, lpre);
 
-    // damping as in the velocity sweep; split groups (XX, YY, ZZ: x,y,z;
-    // XY: x,y; XZ: x,z; YZ: y,z) are contiguous, `sstr` bytes apart
+    // damping as in the velocity sweep; splits per component: XX, YY, ZZ:
+    // x, y, z; XY: x, y; XZ: x, z; YZ: y, z
     constexpr bool PML = DM != 0, DX = (DM & 1) != 0, DY = (DM & 2) != 0, DZ = (DM & 4) != 0;
     const bool px = DX && in_pml(t.i, g.nx, g.pml);
     const bool py = DY && in_pml(j0, g.ny, g.pml);
     C A0 = (C)1, B0 = (C)1, A1[NY], B1[NY];
-    const unsigned sstr = (unsigned)(g.nslots * (long long)sizeof(C));
     // step number keying stochastic rounding (16-bit storage only)
     const uint32_t step = (sizeof(T) == 2 && s.round_mode) ? (uint32_t)gld(s.d_step, 0u) : 0u;
     if (DX) {
@@ -865,15 +863,15 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                     }
                     if (DY && py) {
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) sn3[r][c][1] = bld(s.ss[XX_X + 3 * c], o + sstr);
-                        s2[r][0][1] = bld(s.ss[XY_X], o + sstr);
+                        for (int c = 0; c < 3; ++c) sn3[r][c][1] = bld(s.ss[XX_X + 3 * c + 1], o);
+                        s2[r][0][1] = bld(s.ss[XY_Y], o);
                         s2[r][2][0] = bld(s.ss[YZ_Y], o);
                     }
                     if (DZ && pz) {
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) sn3[r][c][2] = bld(s.ss[XX_X + 3 * c], o + 2 * sstr);
-                        s2[r][1][1] = bld(s.ss[XZ_X], o + sstr);
-                        s2[r][2][1] = bld(s.ss[YZ_Y], o + sstr);
+                        for (int c = 0; c < 3; ++c) sn3[r][c][2] = bld(s.ss[XX_X + 3 * c + 2], o);
+                        s2[r][1][1] = bld(s.ss[XZ_Z], o);
+                        s2[r][2][1] = bld(s.ss[YZ_Z], o);
                     }
                 }
             }
@@ -992,15 +990,15 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 }
                 if (DY && py) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) bst(s.ss[XX_X + 3 * c], o + sstr, sn[c][1]);
-                    bst(s.ss[XY_X], o + sstr, t2[0][1]);
+                    for (int c = 0; c < 3; ++c) bst(s.ss[XX_X + 3 * c + 1], o, sn[c][1]);
+                    bst(s.ss[XY_Y], o, t2[0][1]);
                     bst(s.ss[YZ_Y], o, t2[2][0]);
                 }
                 if (DZ && pz) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) bst(s.ss[XX_X + 3 * c], o + 2 * sstr, sn[c][2]);
-                    bst(s.ss[XZ_X], o + sstr, t2[1][1]);
-                    bst(s.ss[YZ_Y], o + sstr, t2[2][1]);
+                    for (int c = 0; c < 3; ++c) bst(s.ss[XX_X + 3 * c + 2], o, sn[c][2]);
+                    bst(s.ss[XZ_Z], o, t2[1][1]);
+                    bst(s.ss[YZ_Z], o, t2[2][1]);
                 }
             }
         }

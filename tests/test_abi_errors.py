@@ -48,6 +48,16 @@ def test_create_rejects_bad_configurations(L, kw, needle):
     assert needle in L.sg_last_error()
 
 
+def test_tiled_split_arrays_limited_to_4gib_each(L):
+    # 16384x16384 plane, P8: each split array holds 2*8*2^28 z-slab slots,
+    # 16 GiB in FP32, beyond the tiled sweeps' 32-bit offsets
+    h = ctypes.c_void_p(1)
+    c = _cfg(nx=16384, ny=16384, nz=64)
+    assert L.sg_create(ctypes.byref(c), ctypes.byref(h)) == -6  # SG_ERR_UNSUPPORTED
+    assert h.value is None
+    assert b"split" in L.sg_last_error()
+
+
 def test_create_rejects_null_arguments(L):
     h = ctypes.c_void_p()
     assert L.sg_create(None, ctypes.byref(h)) == SG_ERR_INVALID

@@ -12,6 +12,10 @@
   200 cells, so a 440x440x330 grid with the source at the same distance from
   its top boundary must hold the same numbers, bit for bit (64-bit indexing
   of the large arrays, the source above linear index 2^31).
+* 1800x1800x1300 with P24 (4.2 G points, 367 M split slots per split
+  array, so each component's three splits together span more than 4 GiB in
+  FP32): the same comparison against a 460x460x330 grid.  Every split array
+  is addressed from its own base, and only one array has to stay below 4 GiB.
 * 512x512x384, size-independent property: the update is linear in the
   source amplitude and scaling by 2 is exact in binary floating point, so a
   doubled wavelet must give doubled fields.  Where the leading edge of the
@@ -137,9 +141,17 @@ def test_layered512_linearity(mods):
         assert e < 1e-6
 
 
-def test_grid_beyond_2p31_points_matches_small_grid(mods):
-    sgfd, _ = mods
-    pml, nt, h, dt = 10, 20, 10.0, 1e-3
+def _split_slots(nx, ny, nz, p):
+    """Slots of one split array: z slabs, y slabs of the z interior, x slabs
+    of the y/z interior (the solver's split store, 64-element aligned)."""
+    a = lambda n: (n + 63) // 64 * 64
+    ux, uy, uz = nx - 10, ny - 10, nz - 10
+    return (a(2 * p * ny * nx) + a((uz - 2 * p) * 2 * p * nx)
+            + 2 * a((uz - 2 * p) * (uy - 2 * p) * p))
+
+
+def _compare_with_small_grid(sgfd, pml, big_shape, big_src, small_shape, small_src):
+    nt, h, dt = 20, 10.0, 1e-3
     rho, vp = 2000.0, 3000.0
     mu_v, lam_v = rho * (vp / np.sqrt(3)) ** 2, rho * vp ** 2 - 2 * rho * (vp / np.sqrt(3)) ** 2
     w = sgfd.ricker(15.0, dt, nt, "none")
@@ -161,12 +173,9 @@ def test_grid_beyond_2p31_points_matches_small_grid(mods):
         finally:
             s.close()
 
-    big_shape, big_src = (1344, 1344, 1300), (672, 672, 1200)
-    small_shape, small_src = (440, 440, 330), (220, 220, 230)
-    assert np.prod(big_shape) > 2 ** 31 and (big_src[2] * big_shape[1] + big_src[1]) * big_shape[0] > 2 ** 31
-    _say("small grid 440x440x330")
+    _say("small grid %dx%dx%d" % small_shape)
     small = run(*small_shape, small_src)
-    _say("large grid 1344x1344x1300 (2.35 G points)")
+    _say("large grid %dx%dx%d (%.2f G points)" % (*big_shape, np.prod(big_shape) / 1e9))
     big = run(*big_shape, big_src)
     lo = [big_src[a] - small_src[a] for a in range(3)]
     sl = (slice(lo[2], lo[2] + small_shape[2]), slice(lo[1], lo[1] + small_shape[1]), slice(lo[0], lo[0] + small_shape[0]))
@@ -175,3 +184,19 @@ def test_grid_beyond_2p31_points_matches_small_grid(mods):
         assert np.array_equal(big[f][sl], small[f]), f
         # nothing outside the compared block was reached
         assert np.count_nonzero(big[f]) == np.count_nonzero(small[f]), f
+        del big[f]
+
+
+def test_grid_beyond_2p31_points_matches_small_grid(mods):
+    sgfd, _ = mods
+    big_shape, big_src = (1344, 1344, 1300), (672, 672, 1200)
+    assert np.prod(big_shape) > 2 ** 31 and (big_src[2] * big_shape[1] + big_src[1]) * big_shape[0] > 2 ** 31
+    _compare_with_small_grid(sgfd, 10, big_shape, big_src, (440, 440, 330), (220, 220, 230))
+
+
+def test_split_store_beyond_4gib_per_component_matches_small_grid(mods):
+    sgfd, _ = mods
+    big_shape, pml = (1800, 1800, 1300), 24
+    slots = _split_slots(*big_shape, pml)
+    assert 3 * slots * 4 > 2 ** 32 > slots * 4
+    _compare_with_small_grid(sgfd, pml, big_shape, (900, 900, 1200), (460, 460, 330), (230, 230, 230))
