@@ -103,3 +103,17 @@ def test_cpp_dropin_header_compiles_and_links(sgfd_mod, tmp_path):
     with a plain C++ compiler against the C ABI (no HIP headers needed)."""
     exe = _build_example(tmp_path)
     assert exe.exists()
+
+
+def test_cpu_class_dropin_header_compiles_and_links(sgfd_mod, tmp_path):
+    """The reference CPU main.cpp, ported to include/ElasticWaveEquation3D.hpp
+    (the CPU driver class: interior GRID, float*** models), builds with a
+    plain C++ compiler against the C ABI."""
+    import subprocess
+
+    exe = tmp_path / "main_cpu"
+    lib_dir = os.path.join(ROOT, "staggered-grid_amd", "sgfd")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "staggered-grid_amd", "examples", "main_cpu.cpp"),
+                    "-L" + lib_dir, "-lsgfd", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
+    assert exe.exists()
