@@ -579,6 +579,105 @@ class ElasticWaveEquation:
         self.solver.close()
 
 
+class ElasticWaveEquation3D:
+    """Mirror of the CPU driver class (3DCPUwithPML_3DArray_GridIncludePMLandOrder/
+    ElasticWaveEquation3D.h:11-52).  GRID is the interior extent; buoyancy,
+    lambda_ and mu span L + 2*pml + 10 points per axis and are indexed
+    [i][j][k] like the reference's float*** (numpy shape (nx, ny, nz), k
+    fastest); the source is in total-grid indices; the defaults are the
+    reference's (5 m, 0.5 ms, 30 Hz).  The numerics are the CPU reference's:
+    the DIFF_10 literals, its damping formula (tau 3.4, Vs 2000, R 0.001 as
+    float constants), the Ricker with the ceil delay and the transient source.
+    propagate() writes curveLine1..3.txt, snapshot_<q>_<it>.txt and
+    dampingX/Y/Z.txt (ElasticWaveEquation3D.cpp:53-112,302-311) into out_dir,
+    the same text as include/ElasticWaveEquation3D.hpp writes."""
+
+    def __init__(self, grid, timeLength, pml, buoyancy, lambda_, mu, sourceLocation, deltaX=5.0, deltaY=5.0,
+                 deltaZ=5.0, deltaT=0.0005, mainFrequncy=30.0, precision="fp32", kernel="tiled"):
+        self.grid = GRID(*grid)
+        self.timeLength = int(timeLength)
+        self.pml = int(pml)
+        self.n = tuple(int(L) + 2 * self.pml + 10 for L in self.grid)
+        self.model = (buoyancy, lambda_, mu)
+        for a in self.model:
+            if np.shape(a) != self.n:
+                raise ValueError(f"model arrays must have shape (nx, ny, nz) = {self.n}")
+        self.sourceLocation = SOURCELOCATION(*sourceLocation)
+        self.deltaT = deltaT
+        self.mainFrequncy = mainFrequncy
+        f32 = lambda x: float(np.float32(x))
+        nx, ny, nz = self.n
+        self.solver = Solver(nx, ny, nz, self.pml, deltaT, deltaX, deltaY, deltaZ, coef=diff_coef("reference"),
+                             time_length=self.timeLength, precision=precision, source_mode="transient",
+                             kernel=kernel)
+        self.damping = [damping_profile(m, self.pml, f32(h), "reference", f32(2000.0), f32(0.001), f32(3.4))
+                        for m, h in zip(self.n, (deltaX, deltaY, deltaZ))]
+        self.solver.set_damping(*self.damping)
+        if self.timeLength > 0:
+            self.solver.set_source(*self.sourceLocation, ricker(mainFrequncy, deltaT, self.timeLength, "ceil"))
+        e = self.pml + 10
+        sx, sy, sz = self.sourceLocation
+        self.solver.set_stations(np.array([[e, sy, sz], [sx, e, sz], [sx, sy, e]], np.int32), comps=("sxx",))
+        self.curveLineData = None
+        self._propagated = False
+
+    def sourceFunction(self, timeIndex):
+        """Ricker of the reference (ElasticWaveEquation3D.cpp:114-119)."""
+        return float(ricker(self.mainFrequncy, self.deltaT, timeIndex + 1, "ceil")[timeIndex])
+
+    def propagate(self, out_dir=".", write_files=True):
+        """The time loop (ElasticWaveEquation3D.cpp:53-85); returns the three
+        curves.  Runs once per object (the reference would restart its time
+        index on the live fields)."""
+        if self._propagated:
+            raise RuntimeError("propagate() runs once per object")
+        self._propagated = True
+        self.solver.set_medium(*[np.ascontiguousarray(np.asarray(a, np.float32).transpose(2, 1, 0))
+                                 for a in self.model])
+        if write_files:
+            self.calculateDamping(out_dir)
+        done = 0
+        for it in range(self.timeLength):
+            if ((it + 1) >= 200 and it % 50 == 0) or it + 1 == 300:
+                self.solver.step(it + 1 - done)
+                done = it + 1
+                if write_files:
+                    self.snapshoot(it + 1, out_dir)
+        if done < self.timeLength:
+            self.solver.step(self.timeLength - done)
+        tr = self.solver.stations().reshape(-1, 3)
+        self.curveLineData = [tr[:, c].copy() for c in range(3)]
+        if write_files:
+            for c in range(3):
+                ElasticWaveEquation._write_lines(os.path.join(out_dir, f"curveLine{c + 1}.txt"), tr[:, c])
+        return self.curveLineData
+
+    def snapshoot(self, it, out_dir="."):
+        """sxx over the whole grid, i outermost, k innermost; snapshot_1..5
+        created empty (ElasticWaveEquation3D.cpp:87-112)."""
+        F = self.solver.field("sxx")  # (nz, ny, nx)
+        for q in range(1, 6):
+            open(os.path.join(out_dir, f"snapshot_{q}_{it}.txt"), "w").close()
+        ElasticWaveEquation._write_lines(os.path.join(out_dir, f"snapshot_0_{it}.txt"), F.transpose(2, 1, 0))
+
+    def calculateDamping(self, out_dir="."):
+        """The three damping volumes, i outermost (ElasticWaveEquation3D.cpp:
+        253-311): d_a(index on axis a) inside the updated box, 0 on the halo."""
+        upd = []
+        for m in self.n:
+            u = np.zeros(m, bool)
+            u[5:m - 5] = True
+            upd.append(u)
+        box = upd[0][:, None, None] & upd[1][None, :, None] & upd[2][None, None, :]
+        for a, name in enumerate("XYZ"):
+            line = self.damping[a].astype(np.float32).reshape([-1 if b == a else 1 for b in range(3)])
+            ElasticWaveEquation._write_lines(os.path.join(out_dir, f"damping{name}.txt"),
+                                             np.where(box, line, np.float32(0)))
+
+    def close(self):
+        self.solver.close()
+
+
 def scales_for_maxabs(maxabs, headroom_log2=3, target_log2=14):
     """Power-of-two storage scales that map each field's largest |value| to
     at most 2^(target_log2 - headroom_log2) (fields that stayed 0 get 1)."""

@@ -13,7 +13,9 @@ and the ported main program, staggered-grid_amd/examples/main_cpu.cpp.
   300 steps.  The curve files and the snapshots after steps 201, 251 and 300
   must be the same text as a run of the Python mirror on the same model in
   its x-fastest layout, which catches any axis mix-up in the float*** path.
-  snapshot_1..5 must exist and be empty, as the reference leaves them.
+  snapshot_1..5 must exist and be empty, as the reference leaves them.  The
+  Python mirror of the class (sgfd.ElasticWaveEquation3D) must write the
+  same files byte for byte.
 """
 import os
 import subprocess
@@ -134,3 +136,17 @@ def test_heterogeneous_model_matches_python_mirror(exe, tmp_path):
             assert os.path.getsize(tmp_path / f"snapshot_{q}_{step}.txt") == 0
     assert sorted(p.name for p in tmp_path.glob("snapshot_0_*")) == ["snapshot_0_201.txt", "snapshot_0_251.txt",
                                                                       "snapshot_0_300.txt"]
+
+    # the Python mirror of the class writes the same files, byte for byte
+    py = tmp_path / "py"
+    py.mkdir()
+    ewq = sgfd.ElasticWaveEquation3D(L, nt, pml, *[a.transpose(2, 1, 0) for a in (b, lam, mu)], src)
+    try:
+        curves = ewq.propagate(str(py))
+    finally:
+        ewq.close()
+    assert np.array_equal(np.stack(curves, 1), tr)
+    names = sorted(p.name for p in py.iterdir())
+    assert names == sorted(p.name for p in tmp_path.glob("*.txt")) and len(names) == 3 + 3 + 18
+    for name in names:
+        assert (py / name).read_bytes() == (tmp_path / name).read_bytes(), name
