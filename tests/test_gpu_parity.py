@@ -277,6 +277,44 @@ def test_no_pml_against_oracle(oracle_mod, sgfd_mod, kernel):
     assert _rel(tr, trO) < TOL32
 
 
+
+# (nx, ny, nz, pml, source): one updated point; a grid that is all PML
+# (2*pml = n - 10, no interior box); x one column past two and one tiles;
+# fewer x columns than a tile; y all PML with two rows; thin z
+EXTREME = [(11, 11, 11, 0, (5, 5, 5)), (30, 30, 30, 10, (12, 15, 17)), (129, 17, 14, 2, (100, 8, 6)),
+           (65, 23, 19, 3, (40, 11, 9)), (37, 90, 21, 4, (18, 60, 10)), (75, 12, 80, 1, (40, 6, 50))]
+
+
+@pytest.mark.parametrize("kernel", ["tiled", "simple"])
+@pytest.mark.parametrize("geom", EXTREME, ids=lambda g: "x".join(map(str, g[:3])) + f"p{g[3]}")
+def test_extreme_shapes_against_oracle(oracle_mod, sgfd_mod, geom, kernel):
+    """Degenerate and ragged grids (the extremes sg_create accepts), a
+    heterogeneous medium, 40 steps: every field and the station traces
+    rel-L2 <= 1e-5 against the oracle, and the halo stays exactly zero."""
+    O = oracle_mod
+    nx, ny, nz, pml, src = geom
+    nt = 40
+    rng = np.random.default_rng(nx * ny + nz)
+    vp = 3000.0 * (1 + 0.05 * rng.standard_normal((nz, ny, nx)))
+    rho = 2000.0 + 100.0 * rng.standard_normal((nz, ny, nx))
+    mu = (rho * (vp / 1.8) ** 2).astype(np.float32)
+    lam = (rho * vp ** 2 - 2 * rho * (vp / 1.8) ** 2).astype(np.float32)
+    b = (1 / rho).astype(np.float32)
+    p = O.OracleParams(nx, ny, nz, pml, 8e-4, 10.0, 10.0, 10.0, nt, src, coef=O.REF_COEF,
+                       damp_mode=O.DAMP_REFERENCE, damp_v=3000.0)
+    w = O.ricker(20.0, 8e-4, nt, O.RICKER_NO_DELAY)
+    st = np.array([src, [5, 5, 5], [nx - 6, ny - 6, nz - 6]], np.int32)
+    _say(f"extreme {nx}x{ny}x{nz} P{pml} {kernel}")
+    tr, F = _solve(sgfd_mod, p, b, lam, mu, w, st, sgfd_mod.FIELDS, kernel=kernel, damp=_damp_for(O, p))
+    trO, FO = O.run(p, b, lam, mu, w, st)
+    assert np.abs(FO).max() > 0
+    for f in range(9):
+        assert _rel(F[f], FO[f]) < TOL32, sgfd_mod.FIELDS[f]
+        assert not F[f][:5].any() and not F[f][-5:].any()
+        assert not F[f][:, :5].any() and not F[f][:, -5:].any()
+        assert not F[f][:, :, :5].any() and not F[f][:, :, -5:].any()
+    assert _rel(tr, trO) < TOL32
+
 def test_graph_replay_matches_eager(oracle_mod, sgfd_mod):
     O = oracle_mod
     meta, g, p, b, lam, mu, w = _golden_case(O, "g1_workload_hetero.npz")

@@ -38,7 +38,8 @@ def test_benchmark_geometry(checker):
 
 @pytest.mark.parametrize("geom", [(28, 26, 24, 3), (150, 77, 61, 3), (100, 100, 100, 10), (64, 64, 80, 6),
                                   (130, 72, 64, 6), (300, 200, 100, 10), (11, 11, 11, 0), (140, 140, 140, 0),
-                                  (200, 140, 90, 40)])
+                                  (200, 140, 90, 40), (30, 30, 30, 10), (65, 23, 19, 3), (129, 17, 14, 2),
+                                  (37, 90, 21, 4), (75, 12, 80, 1)])
 def test_small_and_ragged_geometries(checker, geom):
     for ty, pty, zc in ((4, 4, 1), (8, 4, 5), (16, 8, 13), (16, 16, 64), (4, 16, 200)):
         _check(checker, *geom, ty, pty, zc)
