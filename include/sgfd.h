@@ -121,6 +121,11 @@ int sg_destroy(sg_solver *s);
  * ElasticMacro.h:379-384, copied in like run()'s H2D at .cu:145-147).
  * x-fastest over the total grid, n = nx*ny*nz values each. */
 int sg_set_medium(sg_solver *s, const float *buoyancy, const float *lambda, const float *mu);
+/* The model as P velocity, S velocity and density (the reference's main
+ * programs form b = 1/rho, lambda and mu from a velocity and a density,
+ * CPUPML/main.cpp:34-36, CONCISE/main.cu:53-55): sg_elastic_from_velocity
+ * below, then sg_set_medium. */
+int sg_set_medium_velocity(sg_solver *s, const float *vp, const float *vs, const float *rho);
 
 /* Per-axis damping d(i) [1/s] for i in [0, n) of that axis (values outside
  * the PML slabs are ignored).  The update factors (1 -/+ dt*d/2) are formed
@@ -283,6 +288,12 @@ int sg_gaussian_field(int nx, int ny, int nz, double corr, unsigned long long se
  * floats. */
 int sg_read_raw_f32(const char *path, long long n, float *out);
 int sg_write_raw_f32(const char *path, long long n, const float *in);
+
+/* b = 1/rho, mu = rho*vs^2, lambda = rho*vp^2 - 2*mu for n points, formed in
+ * double and rounded once to float.  rho and vs must be positive and finite,
+ * vp finite. */
+int sg_elastic_from_velocity(long long n, const float *vp, const float *vs, const float *rho, float *buoyancy,
+                             float *lambda, float *mu);
 
 /* ---- random media on the GPU (spectral method) ------------------------------ */
 

@@ -175,6 +175,24 @@ extern "C" int sg_gaussian_field(int nx, int ny, int nz, double corr, unsigned l
 // ---------------------------------------------------------------------------
 #include <cstdio>
 
+extern "C" int sg_elastic_from_velocity(long long n, const float *vp, const float *vs, const float *rho,
+                                        float *b, float *lam, float *mu)
+{
+    if (n < 0 || !vp || !vs || !rho || !b || !lam || !mu)
+        return sg::fail(SG_ERR_INVALID, "sg_elastic_from_velocity: invalid arguments");
+    for (long long i = 0; i < n; ++i) {
+        if (!(rho[i] > 0) || !std::isfinite(rho[i]) || !(vs[i] > 0) || !std::isfinite(vs[i]) || !std::isfinite(vp[i]))
+            return sg::fail(SG_ERR_INVALID, "sg_elastic_from_velocity: rho and vs must be positive and finite, vp "
+                                            "finite (first bad index " + std::to_string(i) + ")");
+        const double r = rho[i], s = vs[i], p = vp[i];
+        const double m = r * s * s;
+        b[i] = (float)(1.0 / r);
+        mu[i] = (float)m;
+        lam[i] = (float)(r * p * p - 2.0 * m);
+    }
+    return SG_OK;
+}
+
 extern "C" int sg_read_raw_f32(const char *path, long long n, float *out)
 {
     if (!path || !out || n < 1) return sg::fail(SG_ERR_INVALID, "sg_read_raw_f32: invalid arguments");

@@ -548,6 +548,22 @@ int sg_set_medium(sg_solver *s, const float *b, const float *lam, const float *m
     return fill_vs(s);
 }
 
+int sg_set_medium_velocity(sg_solver *s, const float *vp, const float *vs, const float *rho)
+{
+    if (int rc = check_solver(s)) return rc;
+    const long long N = (long long)s->g.nx * s->g.ny * s->g.nz;
+    std::vector<float> b, lam, mu;
+    try {
+        b.resize((size_t)N);
+        lam.resize((size_t)N);
+        mu.resize((size_t)N);
+    } catch (const std::bad_alloc &) {
+        return fail(SG_ERR_NOMEM, "sg_set_medium_velocity: host allocation failed");
+    }
+    if (int rc = sg_elastic_from_velocity(N, vp, vs, rho, b.data(), lam.data(), mu.data())) return rc;
+    return sg_set_medium(s, b.data(), lam.data(), mu.data());
+}
+
 int sg_set_field_scales(sg_solver *s, const float scales[9])
 {
     if (int rc = check_solver(s)) return rc;

@@ -78,6 +78,7 @@ API = {
     "sg_create": (ctypes.c_int, [ctypes.POINTER(_Config), ctypes.POINTER(_VP)]),
     "sg_destroy": (ctypes.c_int, [_VP]),
     "sg_set_medium": (ctypes.c_int, [_VP, _FP, _FP, _FP]),
+    "sg_set_medium_velocity": (ctypes.c_int, [_VP, _FP, _FP, _FP]),
     "sg_set_damping": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "sg_set_source": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP, ctypes.c_int]),
     "sg_set_source_sample": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_float]),
@@ -118,6 +119,7 @@ API = {
     "sg_gaussian_field": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_ulonglong, ctypes.c_double, _FP]),
     "sg_read_raw_f32": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_longlong, _FP]),
+    "sg_elastic_from_velocity": (ctypes.c_int, [ctypes.c_longlong, _FP, _FP, _FP, _FP, _FP, _FP]),
     "sg_write_raw_f32": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_longlong, _FP]),
     "sg_random_medium": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Spectrum),
                                         ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double, _FP]),
@@ -176,6 +178,17 @@ def gaussian_field(nx, ny, nz, corr, seed, clip=4.0) -> np.ndarray:
     out = np.zeros((nz, ny, nx), np.float32)
     _check(lib().sg_gaussian_field(nx, ny, nz, float(corr), int(seed), float(clip), _fptr(out)))
     return out
+
+
+def elastic_from_velocity(vp, vs, rho):
+    """(b, lambda, mu) as float32 arrays of the input shape: b = 1/rho,
+    mu = rho vs^2, lambda = rho vp^2 - 2 mu, formed in double by the library."""
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho)]
+    if not (arrs[0].shape == arrs[1].shape == arrs[2].shape):
+        raise ValueError("vp, vs and rho must have one shape")
+    out = [np.empty_like(arrs[0]) for _ in range(3)]
+    _check(lib().sg_elastic_from_velocity(arrs[0].size, *[_fptr(a) for a in arrs], *[_fptr(o) for o in out]))
+    return tuple(out)
 
 
 def read_vdat(path, nx, ny, nz) -> np.ndarray:
@@ -277,6 +290,11 @@ class Solver:
     def set_medium(self, b, lam, mu):
         arrs = [np.ascontiguousarray(a, np.float32).reshape(self.shape) for a in (b, lam, mu)]
         _check(lib().sg_set_medium(self._h, *[_fptr(a) for a in arrs]))
+
+    def set_medium_velocity(self, vp, vs, rho):
+        """The model as P velocity, S velocity and density (x-fastest)."""
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(self.shape) for a in (vp, vs, rho)]
+        _check(lib().sg_set_medium_velocity(self._h, *[_fptr(a) for a in arrs]))
 
     def set_damping(self, dx, dy, dz):
         arrs = [np.ascontiguousarray(a, np.float64) for a in (dx, dy, dz)]

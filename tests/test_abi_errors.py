@@ -74,6 +74,7 @@ def test_create_rejects_null_arguments(L):
     ("sg_set_rounding", (0,)),
     ("sg_set_compensation", (0,)),
     ("sg_set_source_sample", (0, 1.0)),
+    ("sg_set_medium_velocity", (None, None, None)),
 ])
 def test_null_solver_is_rejected(L, name, args):
     fn = getattr(L, name)
@@ -88,3 +89,31 @@ def test_destroy_null_is_a_no_op(L):
 def test_batch_needs_shots(L):
     out = ctypes.c_void_p()
     assert L.sg_batch_create(None, 0, ctypes.byref(out)) == SG_ERR_INVALID
+
+
+def test_elastic_from_velocity_matches_the_formula_bit_for_bit(L):
+    import numpy as np
+
+    import sgfd
+    from sgfd import configs
+
+    rng = np.random.default_rng(3)
+    vp = (3000 * (1 + 0.1 * rng.standard_normal(5000))).astype(np.float32)
+    vs = (vp / np.float32(1.75)).astype(np.float32)
+    rho = (310 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    got = sgfd.elastic_from_velocity(vp, vs, rho)
+    exp = configs.elastic_from_velocity(vp, vs, rho)
+    for g, e in zip(got, exp):
+        assert g.dtype == np.float32 and np.array_equal(g, e)
+
+
+@pytest.mark.parametrize("bad", ["rho0", "vs0", "vpnan"])
+def test_elastic_from_velocity_rejects_bad_values(L, bad):
+    import numpy as np
+
+    import sgfd
+
+    vp, vs, rho = (np.full(10, v, np.float32) for v in (3000.0, 1700.0, 2000.0))
+    {"rho0": rho, "vs0": vs, "vpnan": vp}[bad][7] = np.nan if bad == "vpnan" else 0.0
+    with pytest.raises(sgfd.SgError, match="first bad index 7"):
+        sgfd.elastic_from_velocity(vp, vs, rho)

@@ -315,6 +315,31 @@ def test_extreme_shapes_against_oracle(oracle_mod, sgfd_mod, geom, kernel):
         assert not F[f][:, :, :5].any() and not F[f][:, :, -5:].any()
     assert _rel(tr, trO) < TOL32
 
+
+def test_medium_from_velocity_matches_medium(sgfd_mod):
+    """sg_set_medium_velocity (Vp, Vs, rho) gives the same run, bit for bit,
+    as sg_set_medium with the moduli the library forms from them."""
+    sgfd = sgfd_mod
+    nx, ny, nz, pml, nt = 60, 50, 40, 4, 30
+    rng = np.random.default_rng(11)
+    vp = (3000 * (1 + 0.05 * rng.standard_normal((nz, ny, nx)))).astype(np.float32)
+    vs = (vp / np.float32(1.75)).astype(np.float32)
+    rho = (310 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    out = []
+    for use_velocity in (True, False):
+        s = sgfd.Solver(nx, ny, nz, pml, 8e-4, 10.0, 10.0, 10.0, time_length=nt)
+        try:
+            if use_velocity:
+                s.set_medium_velocity(vp, vs, rho)
+            else:
+                s.set_medium(*sgfd.elastic_from_velocity(vp, vs, rho))
+            s.set_source(30, 25, 20, sgfd.ricker(20.0, 8e-4, nt, "none"))
+            s.step(nt)
+            out.append(s.fields())
+        finally:
+            s.close()
+    assert np.abs(out[0]).max() > 0 and np.array_equal(out[0], out[1])
+
 def test_graph_replay_matches_eager(oracle_mod, sgfd_mod):
     O = oracle_mod
     meta, g, p, b, lam, mu, w = _golden_case(O, "g1_workload_hetero.npz")
