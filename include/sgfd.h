@@ -253,7 +253,11 @@ int sg_copy_bandwidth(long long bytes, int reps, double *gbps);
 
 /* Groups existing FP32 solvers (any sizes) so one launch per half-step
  * advances all of them over a cross-shot tile work-list.  The solvers stay
- * owned by the caller and must outlive the batch. */
+ * owned by the caller and must outlive the batch.  A shot's source,
+ * stations, damping or medium may change between batch steps: each
+ * sg_batch_step re-reads the shots' descriptors and re-uploads them when
+ * they changed, after any work queued on the shots' own streams.  Before a
+ * shot is stepped or read on its own, call sg_batch_synchronize. */
 int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out);
 /* Launch shape of the batched sweeps: box shapes (as sg_set_tile; z_chunk
  * must be > 0), grid size (0 = one block per work item; smaller = persistent

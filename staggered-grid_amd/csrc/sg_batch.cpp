@@ -46,6 +46,9 @@ struct sg_batch {
     int grid = 0;
     Tile tile{4, 1, 4, 1, 24};
     int max_gather = 0;
+    // the shot and auxiliary descriptors as last uploaded to d_shots/d_aux
+    std::vector<unsigned char> shot_host;
+    std::vector<AuxDesc> aux_host;
     bool use_graph = true;
     bool graph_valid = false;
     hipGraph_t graph = nullptr;
@@ -76,6 +79,37 @@ int build_items(sg_batch *b)
     SG_HIP(hipMalloc(&b->d_items, all.size() * sizeof(WorkItem)));
     SG_HIP(hipMemcpy(b->d_items, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     b->graph_valid = false;
+    return SG_OK;
+}
+
+// Upload the shots' descriptors when they differ from the last upload (or
+// when forced).  They hold device pointers that sg_set_stations and
+// sg_set_source reallocate and sg_set_damping_local adds, so every step
+// checks them; kernels still in flight read the old copy, hence the wait.
+// The captured graph is dropped: the gather launch depends on the stations.
+int sync_descriptors(sg_batch *b, bool force)
+{
+    const size_t n = b->shots.size(), ss = sizeof(Shot<float, float>);
+    std::vector<unsigned char> sh(n * ss);
+    std::vector<AuxDesc> aux(n);
+    std::memset(static_cast<void *>(aux.data()), 0, n * sizeof(AuxDesc));  // padding compares equal
+    int max_gather = 0;
+    for (size_t i = 0; i < n; ++i) {
+        std::memcpy(sh.data() + i * ss, solver_shot(b->shots[i]), ss);
+        solver_aux(b->shots[i], &aux[i]);
+        max_gather = std::max(max_gather, aux[i].nst * aux[i].ncomp);
+    }
+    if (!force && sh == b->shot_host &&
+        std::memcmp(static_cast<const void *>(aux.data()), static_cast<const void *>(b->aux_host.data()),
+                    n * sizeof(AuxDesc)) == 0)
+        return SG_OK;
+    SG_HIP(hipStreamSynchronize(b->stream));
+    SG_HIP(hipMemcpy(b->d_shots, sh.data(), sh.size(), hipMemcpyHostToDevice));
+    SG_HIP(hipMemcpy(b->d_aux, aux.data(), n * sizeof(AuxDesc), hipMemcpyHostToDevice));
+    b->shot_host.swap(sh);
+    b->aux_host.swap(aux);
+    b->max_gather = max_gather;
+    drop_graph(b);
     return SG_OK;
 }
 
@@ -141,27 +175,19 @@ int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out)
         return rc;
     };
     hipError_t e = hipSetDevice(dev);
-    std::vector<unsigned char> sh((size_t)n * sizeof(Shot<float, float>));
-    std::vector<AuxDesc> aux(n);
-    for (int i = 0; i < n && e == hipSuccess; ++i) {
-        e = hipStreamSynchronize(solver_stream(shots[i]));
-        std::memcpy(sh.data() + (size_t)i * sizeof(Shot<float, float>), solver_shot(shots[i]), sizeof(Shot<float, float>));
-        solver_aux(shots[i], &aux[i]);
-        b->max_gather = std::max(b->max_gather, aux[i].nst * aux[i].ncomp);
-    }
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = hipStreamSynchronize(solver_stream(shots[i]));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     for (int q = 0; q < 2 && e == hipSuccess; ++q) {
         e = hipStreamCreateWithFlags(&b->side[q], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming);
     }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMalloc(&b->d_shots, sh.size());
-    if (e == hipSuccess) e = hipMemcpy(b->d_shots, sh.data(), sh.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->d_shots, (size_t)n * sizeof(Shot<float, float>));
     if (e == hipSuccess) e = hipMalloc(&b->d_aux, n * sizeof(AuxDesc));
-    if (e == hipSuccess) e = hipMemcpy(b->d_aux, aux.data(), n * sizeof(AuxDesc), hipMemcpyHostToDevice);
     if (e != hipSuccess)
         return bail(fail(e == hipErrorOutOfMemory ? SG_ERR_NOMEM : SG_ERR_HIP,
                          std::string("sg_batch_create: ") + hipGetErrorString(e)));
+    if (int rc = sync_descriptors(b, true)) return bail(rc);
     if (int rc = build_items(b)) return bail(rc);
     *out = b;
     return SG_OK;
@@ -190,6 +216,9 @@ int sg_batch_step(sg_batch *b, int nsteps)
     if (!b) return fail(SG_ERR_INVALID, "sg_batch_step: NULL batch");
     if (nsteps < 0) return fail(SG_ERR_INVALID, "sg_batch_step: nsteps < 0");
     SG_HIP(hipSetDevice(b->device));
+    // work a shot queued on its own stream comes first
+    for (auto *s : b->shots) SG_HIP(hipStreamSynchronize(solver_stream(s)));
+    if (int rc = sync_descriptors(b, false)) return rc;
     if (b->use_graph) {
         if (!b->graph_valid) {
             drop_graph(b);

@@ -444,7 +444,9 @@ class Solver:
 
 class Batch:
     """Several FP32 solvers advanced by one launch per half-step (sg_batch_*).
-    Configure the shots first; the batch snapshots their source/station set-up."""
+    A shot's source, stations, damping or medium may change between batch
+    steps (the next step picks the change up); call synchronize() before
+    stepping or reading a shot on its own."""
 
     def __init__(self, solvers):
         self.solvers = list(solvers)

@@ -121,3 +121,52 @@ def test_batched_shots_match_single_shots(sgfd_mod, launch):
     finally:
         for s in solvers:
             s.close()
+
+
+def _reconfigure(sgfd, s, wl, k):
+    """Between steps: a new station set (another count and components), a
+    new source point and wavelet, and the per-cell damping; each of these
+    reallocates or adds device buffers behind the shot's descriptors."""
+    st = np.array([[wl.nx // 2, wl.ny // 2, 5 + wl.pml + k], [wl.nx // 3, 12, 20]], np.int32)
+    s.set_stations(st, comps=("vz", "sxx", "syz"))
+    src = (wl.src[0] + 3, wl.src[1] - 2, wl.src[2] + 1)
+    s.set_source(*src, -0.5 * sgfd.ricker(25.0, 1e-3, wl.nt, "none"))
+    s.set_damping_local(*[sgfd.damping_profile(n, wl.pml, 10.0, "cuda", v=1.0, tau=4.3) for n in (wl.nx, wl.ny, wl.nz)])
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_picks_up_shot_changes(sgfd_mod, graph):
+    """Shots reconfigured between batch steps (stations, source, damping)
+    give the same results, bit for bit, as the same shots stepped alone with
+    the same changes at the same step."""
+    sgfd = sgfd_mod
+    wls = [_small_workload(sgfd, 30 + i, *d) for i, d in enumerate([(96, 80, 60), (70, 101, 50)])]
+    half = wls[0].nt // 2
+    _say(f"batch of {len(wls)} shots reconfigured after {half} steps, graph={graph}")
+    ref = []
+    for k, wl in enumerate(wls):
+        s = wl.solver()
+        try:
+            s.step(half)
+            _reconfigure(sgfd, s, wl, k)
+            s.step(wl.nt - half)
+            ref.append((s.stations(), s.fields()))
+        finally:
+            s.close()
+    solvers = [wl.solver() for wl in wls]
+    try:
+        b = sgfd.Batch(solvers)
+        b.set_launch(graph=graph)
+        b.step(half)
+        b.synchronize()
+        for k, (s, wl) in enumerate(zip(solvers, wls)):
+            _reconfigure(sgfd, s, wl, k)
+        b.step(wls[0].nt - half)
+        b.synchronize()
+        for s, (tr0, F0) in zip(solvers, ref):
+            assert np.array_equal(s.fields(), F0)
+            assert np.array_equal(s.stations(), tr0)
+        b.close()
+    finally:
+        for s in solvers:
+            s.close()
