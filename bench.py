@@ -28,7 +28,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, planes=60, steps=4):
+def cpu_baseline(wl, planes=60, steps=10):
     """CPU leg on a bounded sample of the same workload: the first `planes`
     z-planes of the model (same x/y extent and PML), `steps` time steps, one
     thread.  Times the reference's own CPU solver
@@ -67,7 +67,7 @@ def cpu_baseline(wl, planes=60, steps=4):
             "host_cpus": os.cpu_count()}
 
 
-def cpu_all_cores(wl, planes=60, steps=4):
+def cpu_all_cores(wl, planes=60, steps=10):
     """The C restatement of the reference CPU solver with its loop nest
     split over up to 16 host threads, on the same bounded sample as
     cpu_baseline (informative beside the one-thread reference)."""
