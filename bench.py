@@ -212,6 +212,43 @@ def bench_medium(args):
     print(json.dumps(out))
 
 
+def precision_study(args):
+    """The metric's second half: each storage precision runs the full
+    workload (the same source, medium and stations); its Gpt-updates/s
+    (HIP events over the timed steps) and its seismogram rel-L2 against the
+    FP64 build's traces.  16-bit storage uses scales from an FP32 pilot run.
+    Reports the FP16/FP32 speed ratio and whether FP16 meets rel-L2 <= 1e-3."""
+    import numpy as np
+    from sgfd import configs
+
+    wl = configs.BUILDERS[args.workload]()
+    K = wl.nt
+    res = {}
+    ref = None
+    for prec in ("fp64", "fp32", "fp16", "bf16"):
+        log(f"bench: {prec}, {K} steps")
+        s = wl.solver(precision=prec)
+        try:
+            _, _, ms = s.step_timed(K)
+            tr = s.stations().astype(np.float64)
+        finally:
+            s.close()
+        if ref is None:
+            ref = tr
+        err = float(np.linalg.norm(tr - ref) / np.linalg.norm(ref))
+        res[prec] = {"gpt_per_s": wl.points * K / (ms / 1e3) / 1e9, "ms_per_step": ms / K,
+                     "seismogram_rel_l2_vs_fp64": err}
+    ratio = res["fp16"]["gpt_per_s"] / res["fp32"]["gpt_per_s"]
+    out = {"metric": "FP16-store vs FP32 at fixed seismogram error", "value": ratio, "unit": "x (FP16/FP32 Gpt/s)",
+           "n_gpus": 1, "steps": K, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": wl.description, "grid": [wl.nx, wl.ny, wl.nz], "pml": wl.pml,
+                      "stations": int(len(wl.stations)), "components": list(wl.comps)},
+           "target": {"ratio": 1.7, "seismogram_rel_l2": 1e-3},
+           "fp16_meets_error_target": res["fp16"]["seismogram_rel_l2_vs_fp64"] <= 1e-3,
+           "precisions": res}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +265,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
     ap.add_argument("--compare-sequential", action="store_true",
                     help="batched32: also time the shots one solver at a time")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="FP64/FP32/FP16/BF16 over the full workload: speed and seismogram error against FP64")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -247,6 +286,9 @@ def main():
         return
     if args.workload == "medium512":
         bench_medium(args)
+        return
+    if args.precision_study:
+        precision_study(args)
         return
 
     log(f"bench: building workload {args.workload}")
