@@ -340,6 +340,46 @@ def test_medium_from_velocity_matches_medium(sgfd_mod):
             s.close()
     assert np.abs(out[0]).max() > 0 and np.array_equal(out[0], out[1])
 
+
+def _random_case(seed):
+    """A seeded random geometry: extents 11 + 2P .. 11 + 2P + 90, P in
+    0..12, one of the five box shapes for the interior and the PML boxes, a
+    z-chunk of 1..40 planes, the source and three stations anywhere in the
+    updated range."""
+    rng = np.random.default_rng(seed)
+    pml = int(rng.integers(0, 13))
+    dims = [int(rng.integers(11 + 2 * pml, 11 + 2 * pml + 91)) for _ in range(3)]
+    shapes = [(4, 1), (8, 1), (8, 2), (16, 1), (16, 2)]
+    a, b = (shapes[int(i)] for i in rng.integers(0, 5, 2))
+    tile = (a[0], a[1], b[0], b[1], int(rng.integers(1, 41)))
+    pick = lambda: tuple(int(rng.integers(5, n - 5)) for n in dims)
+    return dims, pml, tile, pick(), np.array([pick() for _ in range(3)], np.int32)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_geometries_against_oracle(oracle_mod, sgfd_mod, seed):
+    """Random extents, PML widths, box shapes and z-chunks (seeded), a
+    heterogeneous medium, 30 steps: the tiled sweeps against the oracle,
+    every field and the traces rel-L2 <= 1e-5."""
+    O = oracle_mod
+    (nx, ny, nz), pml, tile, src, st = _random_case(seed)
+    nt = 30
+    rng = np.random.default_rng(100 + seed)
+    vp = 3000.0 * (1 + 0.05 * rng.standard_normal((nz, ny, nx)))
+    rho = 2000.0 + 100.0 * rng.standard_normal((nz, ny, nx))
+    mu = (rho * (vp / 1.8) ** 2).astype(np.float32)
+    lam = (rho * vp ** 2 - 2 * rho * (vp / 1.8) ** 2).astype(np.float32)
+    b = (1 / rho).astype(np.float32)
+    p = O.OracleParams(nx, ny, nz, pml, 8e-4, 10.0, 10.0, 10.0, nt, src, coef=O.EXACT_COEF,
+                       damp_mode=O.DAMP_QUADRATIC, damp_v=3000.0)
+    w = O.ricker(20.0, 8e-4, nt, O.RICKER_NO_DELAY)
+    _say(f"random case {seed}: {nx}x{ny}x{nz} P{pml} tile {tile} source {src}")
+    tr, F = _solve(sgfd_mod, p, b, lam, mu, w, st, sgfd_mod.FIELDS, damp=_damp_for(O, p), tile=tile)
+    trO, FO = O.run(p, b, lam, mu, w, st)
+    for f in range(9):
+        assert _rel(F[f], FO[f]) < TOL32, sgfd_mod.FIELDS[f]
+    assert _rel(tr, trO) < TOL32
+
 def test_graph_replay_matches_eager(oracle_mod, sgfd_mod):
     O = oracle_mod
     meta, g, p, b, lam, mu, w = _golden_case(O, "g1_workload_hetero.npz")

@@ -52,3 +52,12 @@ def test_random_geometries(checker):
         dims = [int(rng.integers(11 + 2 * pml, 11 + 2 * pml + 150)) for _ in range(3)]
         _check(checker, *dims, pml, int(rng.choice([4, 8, 16])), int(rng.choice([4, 8, 16])),
                int(rng.integers(1, 60)))
+
+
+def test_gpu_random_parity_geometries(checker):
+    """The geometries and box shapes of the GPU random-geometry parity test."""
+    from test_gpu_parity import _random_case
+
+    for seed in range(8):
+        (nx, ny, nz), pml, tile, _, _ = _random_case(seed)
+        _check(checker, nx, ny, nz, pml, tile[0], tile[2], tile[4])
