@@ -117,6 +117,7 @@ def bench_batched(args):
     shots = configs.batched_shots(nt=args.warmup + args.steps)
     log(f"bench: shots ready in {time.perf_counter() - t:.1f} s")
     solvers = [wl.solver() for wl in shots]
+    b = None
     try:
         b = sgfd.Batch(solvers)
         if args.tile:
@@ -124,8 +125,9 @@ def bench_batched(args):
         b.step(args.warmup)
         b.synchronize()
         ms = b.step_timed(args.steps)
-        b.close()
     finally:
+        if b is not None:
+            b.close()
         for s in solvers:
             s.close()
     pts = sum(wl.points for wl in shots)

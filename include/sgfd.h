@@ -115,6 +115,8 @@ typedef struct sg_config {
  * device.  Fields start at zero; damping defaults to the reference CPU
  * formula (tau 3.4, V 2000, R 1e-3) until sg_set_damping is called. */
 int sg_create(const sg_config *cfg, sg_solver **out);
+/* Frees the solver (NULL: no-op).  SG_ERR_STATE, and nothing freed, while a
+ * batch holds it: destroy the batch first. */
 int sg_destroy(sg_solver *s);
 
 /* Host-side copy of the model (Medium buoyancy/lambda/mu of CONCISE/

@@ -26,6 +26,7 @@ hipStream_t solver_stream(const sg_solver *s);
 const Geom &solver_geom(const sg_solver *s);
 void solver_aux(const sg_solver *s, AuxDesc *a);
 void solver_add_steps(sg_solver *s, long long n);
+void solver_batch_ref(sg_solver *s, int d);
 int solver_device(const sg_solver *s);
 }  // namespace sg
 
@@ -189,6 +190,7 @@ int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out)
                          std::string("sg_batch_create: ") + hipGetErrorString(e)));
     if (int rc = sync_descriptors(b, true)) return bail(rc);
     if (int rc = build_items(b)) return bail(rc);
+    for (auto *s : b->shots) solver_batch_ref(s, +1);
     *out = b;
     return SG_OK;
 }
@@ -276,6 +278,7 @@ int sg_batch_destroy(sg_batch *b)
     if (!b) return SG_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto *s : b->shots) solver_batch_ref(s, -1);
     free_batch(b);
     delete b;
     return SG_OK;

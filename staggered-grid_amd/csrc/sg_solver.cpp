@@ -123,6 +123,7 @@ struct sg_solver {
     void *comp[NFIELD] = {};  // compensation arrays (16-bit storage, see sg_set_compensation)
     unsigned comp_mask = 0;
     long long steps_done = 0;
+    int batch_refs = 0;  // live batches holding this solver's device pointers
     std::vector<double> damp[3];
     bool damp_local = false;  // damp[] holds g of d = g * sqrt(mu*b) (sg_set_damping_local)
     void *vs = nullptr;       // local S velocity per split slot (compute type)
@@ -501,6 +502,8 @@ int sg_create(const sg_config *cfg, sg_solver **out)
 int sg_destroy(sg_solver *s)
 {
     if (!s) return SG_OK;
+    if (s->batch_refs > 0)
+        return fail(SG_ERR_STATE, "sg_destroy: the solver belongs to a live batch; destroy the batch first");
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     free_all(s);
@@ -1140,6 +1143,7 @@ int solver_device(const sg_solver *s) { return s->device; }
 hipStream_t solver_stream(const sg_solver *s) { return s->stream; }
 const Geom &solver_geom(const sg_solver *s) { return s->g; }
 void solver_add_steps(sg_solver *s, long long n) { s->steps_done += n; }
+void solver_batch_ref(sg_solver *s, int d) { s->batch_refs += d; }
 void solver_aux(const sg_solver *s, AuxDesc *a)
 {
     a->amp_d = s->amp_d;

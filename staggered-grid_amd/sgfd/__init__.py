@@ -266,8 +266,9 @@ class Solver:
 
     # -- lifetime
     def close(self):
+        """Frees the solver; raises SgError while a live Batch holds it."""
         if getattr(self, "_h", None):
-            lib().sg_destroy(self._h)
+            _check(lib().sg_destroy(self._h))
             self._h = None
 
     def __del__(self):

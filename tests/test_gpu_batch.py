@@ -108,6 +108,7 @@ def test_batched_shots_match_single_shots(sgfd_mod, launch):
     _say(f"batch of {len(wls)} shots, launch {launch}")
     ref = [_run(wl) for wl in wls]
     solvers = [wl.solver() for wl in wls]
+    b = None
     try:
         b = sgfd.Batch(solvers)
         b.set_launch(**launch)
@@ -117,8 +118,9 @@ def test_batched_shots_match_single_shots(sgfd_mod, launch):
             assert s.steps_done == wls[0].nt
             assert np.array_equal(s.fields(), F0)
             assert np.array_equal(s.stations(), tr0)
-        b.close()
     finally:
+        if b is not None:
+            b.close()
         for s in solvers:
             s.close()
 
@@ -154,6 +156,7 @@ def test_batch_picks_up_shot_changes(sgfd_mod, graph):
         finally:
             s.close()
     solvers = [wl.solver() for wl in wls]
+    b = None
     try:
         b = sgfd.Batch(solvers)
         b.set_launch(graph=graph)
@@ -166,7 +169,31 @@ def test_batch_picks_up_shot_changes(sgfd_mod, graph):
         for s, (tr0, F0) in zip(solvers, ref):
             assert np.array_equal(s.fields(), F0)
             assert np.array_equal(s.stations(), tr0)
+    finally:
+        if b is not None:
+            b.close()
+        for s in solvers:
+            s.close()
+
+
+def test_destroying_a_batched_solver_is_refused(sgfd_mod):
+    """sg_destroy of a solver that a live batch holds returns SG_ERR_STATE
+    and frees nothing (the batch would keep its device pointers); after the
+    batch is destroyed the solver can be stepped and destroyed as usual."""
+    sgfd = sgfd_mod
+    wls = [_small_workload(sgfd, 40 + i, 64, 48, 40, nt=10) for i in range(2)]
+    solvers = [wl.solver() for wl in wls]
+    b = sgfd.Batch(solvers)
+    try:
+        with pytest.raises(sgfd.SgError, match="live batch"):
+            solvers[0].close()
+        b.step(5)
+        b.synchronize()
+    finally:
         b.close()
+    try:
+        solvers[0].step(5)
+        assert solvers[0].steps_done == 10 and np.abs(solvers[0].fields()).max() > 0
     finally:
         for s in solvers:
             s.close()
