@@ -56,8 +56,8 @@ def test_random_geometries(checker):
 
 def test_gpu_random_parity_geometries(checker):
     """The geometries and box shapes of the GPU random-geometry parity test."""
-    from test_gpu_parity import _random_case
+    from test_gpu_parity import RANDOM_SEEDS, _random_case
 
-    for seed in range(8):
+    for seed in RANDOM_SEEDS:
         (nx, ny, nz), pml, tile, _, _ = _random_case(seed)
         _check(checker, nx, ny, nz, pml, tile[0], tile[2], tile[4])
