@@ -44,8 +44,11 @@ enum {
      * (..._concise/ElasticWaveEquation.cu:216-218 then :256-258), whereas the
      * CPU path rebuilds sigma from its splits every step
      * (ElasticWaveEquation3D.cpp:215,220,225) and so drops the injection after
-     * the velocity update.  This mode reproduces the CUDA behaviour: the
-     * running float sum A += dt*dt*s is added before each velocity update. */
+     * the velocity update.  This mode reproduces the CUDA behaviour: sigma +=
+     * dt*dt*s in float, kept for good at an interior source point (the
+     * amount also goes into one split) and dropped after one velocity update
+     * at a source inside the PML, where the CUDA path also rebuilds sigma
+     * from its splits. */
     ORACLE_INJECT_ACCUMULATE = 2
 };
 

@@ -118,7 +118,12 @@ static int FN(run)(const oracle_params *p, const REAL *b_in, const REAL *lam_in,
     int nthr = p->nthreads > 0 ? p->nthreads : 1;
 #endif
 
-    REAL acc_src = 0;
+    /* the source point lies in the undamped interior box (no PML axis) */
+    int src_interior = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int n = a == 0 ? NX : (a == 1 ? NY : NZ);
+        if (p->src[a] < 5 + P || p->src[a] >= n - 5 - P) src_interior = 0;
+    }
     for (int it = 0; it < p->nt; ++it) {
         /* loadSeismicSource (ElasticWaveEquation3D.cpp:121-130) */
         if (p->inject_mode == ORACLE_INJECT_DOUBLE) {
@@ -127,10 +132,22 @@ static int FN(run)(const oracle_params *p, const REAL *b_in, const REAL *lam_in,
             syy[src] = (REAL)((double)syy[src] + amp);
             szz[src] = (REAL)((double)szz[src] + amp);
         } else if (p->inject_mode == ORACLE_INJECT_ACCUMULATE) {
-            acc_src += dt * dt * wavelet[it];
-            sxx[src] += acc_src;
-            syy[src] += acc_src;
-            szz[src] += acc_src;
+            /* The CUDA path's interior sigma is unsplit (..._concise/
+             * ElasticWaveEquation.cu:256-258), so what locateSource adds
+             * (.cu:216-218) stays in sigma for good; its PML regions rebuild
+             * sigma from the splits (ElasticMacro.h:466-543), which drops the
+             * injection after one velocity update, as here.  In this
+             * split-everywhere form the interior case adds the amount to one
+             * split as well, where the rebuild keeps it. */
+            REAL amp = dt * dt * wavelet[it];
+            sxx[src] += amp;
+            syy[src] += amp;
+            szz[src] += amp;
+            if (src_interior) {
+                S[OR_SXX][0][src] += amp;
+                S[OR_SYY][0][src] += amp;
+                S[OR_SZZ][0][src] += amp;
+            }
         } else {
             REAL amp = dt * dt * wavelet[it];
             sxx[src] += amp;

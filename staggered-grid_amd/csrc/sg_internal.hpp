@@ -141,7 +141,7 @@ struct WorkItem {
 struct AuxDesc {
     const double *amp_d;
     const float *amp_acc;
-    int namp, accumulate;
+    int namp, inject;  // INJECT_*
     int *d_step;
     float *saved;
     const long long *st_idx;
@@ -191,8 +191,14 @@ hipError_t launch_sweep_fused(int which, int prec, int comp, const void *shots_d
 // work-lists: interior, x face, y face, z face, edges/corners
 constexpr int NLIST = 5;
 constexpr int LIST_DM[NLIST] = {0, 1, 2, 4, 7};
+// Source injection modes: TRANSIENT adds pow(dt,2)*s in double and the stress
+// sweep restarts from the saved sigma at the source; ACCUMULATE adds
+// dt*dt*s in the compute type and keeps it; TRANSIENT_F adds dt*dt*s and
+// restarts from the saved sigma.
+enum { INJECT_TRANSIENT = 0, INJECT_ACCUMULATE = 1, INJECT_TRANSIENT_F = 2 };
+
 hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc,
-                         int namp, int accumulate, int *d_step, void *saved, hipStream_t st);
+                         int namp, int mode, int *d_step, void *saved, hipStream_t st);
 hipError_t launch_gather(int prec, const void *shot, const long long *idx, const int *comp, int nst,
                          int ncomp, int interval, long long capacity, void *out, const int *d_step,
                          hipStream_t st);

@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_stress_simple(const Shot<T, C> s)
 // 123-125).  ACCUMULATE: sigma += dt*dt*s(it) in the storage precision
 // (3DGPUwithPML_1DArraySingleGPU_concise/ElasticWaveEquation.cu:216-218).
 template <class T, class C>
-__global__ void k_inject(const Shot<T, C> s, const double *amp_d, const C *amp_acc, int namp, int accumulate,
+__global__ void k_inject(const Shot<T, C> s, const double *amp_d, const C *amp_acc, int namp, int mode,
                          int *d_step, T *saved)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -258,12 +258,12 @@ __global__ void k_inject(const Shot<T, C> s, const double *amp_d, const C *amp_a
         T *pc = (sizeof(T) == 2 && s.comp[f]) ? s.comp[f] + s.src : nullptr;
         const C cur = ldf(s, f, s.src);
         C nv = cur;
-        if (accumulate) {
+        if (mode == INJECT_ACCUMULATE) {
             if (it < namp) nv = cur + amp_acc[it];
         } else {
             saved[c] = *p;
             if (pc) saved[3 + c] = *pc;
-            if (it < namp) nv = (C)((double)cur + amp_d[it]);
+            if (it < namp) nv = mode == INJECT_TRANSIENT_F ? cur + amp_acc[it] : (C)((double)cur + amp_d[it]);
         }
         if (it < namp) {
             const T q = Cvt<T, C>::st(nv, s.fscale[f]);
@@ -303,11 +303,11 @@ __global__ void k_inject_batch(const Shot<float, float> *shots, const AuxDesc *a
     if (s.src < 0) return;
     for (int c = 0; c < 3; ++c) {
         float *p = s.f[SXX + c] + s.src;
-        if (a.accumulate) {
+        if (a.inject == INJECT_ACCUMULATE) {
             if (it < a.namp) *p = *p + a.amp_acc[it];
         } else {
             a.saved[c] = *p;
-            if (it < a.namp) *p = (float)((double)*p + a.amp_d[it]);
+            if (it < a.namp) *p = a.inject == INJECT_TRANSIENT_F ? *p + a.amp_acc[it] : (float)((double)*p + a.amp_d[it]);
         }
     }
 }
@@ -495,12 +495,12 @@ hipError_t launch_stress_simple(int prec, const void *shot, hipStream_t st)
 }
 
 template <class T, class C>
-static hipError_t inject_t(const void *shot, const double *amp_d, const void *amp_acc, int namp, int acc,
+static hipError_t inject_t(const void *shot, const double *amp_d, const void *amp_acc, int namp, int mode,
                            int *d_step, void *saved, hipStream_t st)
 {
     const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
     hipLaunchKernelGGL((k_inject<T, C>), dim3(1), dim3(64), 0, st, s, amp_d, static_cast<const C *>(amp_acc), namp,
-                       acc, d_step, static_cast<T *>(saved));
+                       mode, d_step, static_cast<T *>(saved));
     return hipGetLastError();
 }
 
@@ -529,9 +529,9 @@ hipError_t launch_sweep_fused(int which, int prec, int comp, const void *shots_d
 }
 
 hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const void *amp_acc, int namp,
-                         int accumulate, int *d_step, void *saved, hipStream_t st)
+                         int mode, int *d_step, void *saved, hipStream_t st)
 {
-    SG_DISPATCH(prec, inject_t, shot, amp_d, amp_acc, namp, accumulate, d_step, saved, st)
+    SG_DISPATCH(prec, inject_t, shot, amp_d, amp_acc, namp, mode, d_step, saved, st)
 }
 
 template <class T, class C>

@@ -148,6 +148,26 @@ struct sg_solver {
 
 namespace {
 
+// The source point lies in a PML slab (some axis damps there).
+bool src_in_pml(const sg_solver *s)
+{
+    if (s->src < 0 || s->g.pml == 0) return false;
+    const Geom &g = s->g;
+    const long long i = s->src % g.nx, j = (s->src / g.nx) % g.ny, k = s->src / ((long long)g.nx * g.ny);
+    return in_pml((int)i, g.nx, g.pml) || in_pml((int)j, g.ny, g.pml) || in_pml((int)k, g.nz, g.pml);
+}
+
+// How the inject kernel adds the source (INJECT_*): the CPU path's transient
+// double amount; the CUDA path's float amount kept for good in its unsplit
+// interior; and, for a CUDA-path source inside the PML, where that path
+// rebuilds sigma from its splits (..._concise/ElasticMacro.h:466-543), the
+// float amount dropped after one velocity update.
+int inject_mode(const sg_solver *s)
+{
+    if (s->cfg.source_mode != SG_SOURCE_ACCUMULATE) return INJECT_TRANSIENT;
+    return src_in_pml(s) ? INJECT_TRANSIENT_F : INJECT_ACCUMULATE;
+}
+
 template <class T, class C>
 void fill_shot(sg_solver *s)
 {
@@ -177,8 +197,7 @@ void fill_shot(sg_solver *s)
     }
     h.dt = (C)s->cfg.dt;
     h.src = s->src;
-    h.src_saved = (s->cfg.source_mode == SG_SOURCE_TRANSIENT && s->src >= 0) ? static_cast<const T *>(s->saved)
-                                                                               : nullptr;
+    h.src_saved = (s->src >= 0 && inject_mode(s) != INJECT_ACCUMULATE) ? static_cast<const T *>(s->saved) : nullptr;
     for (int f = 0; f < NFIELD; ++f) {
         h.fscale[f] = s->fscale[f];
         h.finv[f] = 1.0f / s->fscale[f];
@@ -279,8 +298,8 @@ int upload_tables(sg_solver *s)
 
 int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
 {
-    const int acc = s->cfg.source_mode == SG_SOURCE_ACCUMULATE;
-    SG_HIP(launch_inject(s->prec, s->shot, s->amp_d, s->amp_acc, s->namp, acc, s->d_step, s->saved, s->stream));
+    SG_HIP(launch_inject(s->prec, s->shot, s->amp_d, s->amp_acc, s->namp, inject_mode(s), s->d_step, s->saved,
+                         s->stream));
     if (e0) SG_HIP(hipEventRecord(e0, s->stream));
     if (s->cfg.kernel == SG_KERNEL_SIMPLE) {
         SG_HIP(launch_velocity_simple(s->prec, s->shot, s->stream));
@@ -1149,7 +1168,7 @@ void solver_aux(const sg_solver *s, AuxDesc *a)
     a->amp_d = s->amp_d;
     a->amp_acc = static_cast<const float *>(s->amp_acc);
     a->namp = s->namp;
-    a->accumulate = s->cfg.source_mode == SG_SOURCE_ACCUMULATE;
+    a->inject = inject_mode(s);
     a->d_step = s->d_step;
     a->saved = static_cast<float *>(s->saved);
     a->st_idx = s->st_idx;

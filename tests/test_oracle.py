@@ -229,3 +229,35 @@ def test_fp64_oracle_agrees_with_fp32(oracle_mod):
                       w.astype(np.float64), g["stations"], dtype=np.float64)
     assert _rel_l2(F32, F64) < 1e-5
     assert _rel_l2(tr32, tr64) < 1e-5
+
+
+@pytest.mark.parametrize("where", ["interior", "pml"])
+def test_accumulate_injection_closed_form(oracle_mod, where):
+    """With zero buoyancy the velocities stay 0, so sigma at the source is
+    the injection alone.  ACCUMULATE (the CUDA path): at an interior point
+    sxx = syy = szz = the running float sum of dt*dt*s(it) after every step
+    (the unsplit interior keeps each injection); inside the PML the split
+    rebuild drops it, so 0.  The CPU form (DOUBLE) drops it everywhere."""
+    import numpy as np
+
+    O = oracle_mod
+    n, pml, nt, dt = 24, 3, 12, 1e-3
+    src = (12, 12, 12) if where == "interior" else (6, 12, 12)
+    z = np.zeros((n, n, n), np.float32)
+    lam = np.full((n, n, n), 1e9, np.float32)
+    w = O.ricker(20.0, dt, nt, O.RICKER_NO_DELAY)
+    st = np.array([src], np.int32)
+    p = O.OracleParams(n, n, n, pml, dt, 10.0, 10.0, 10.0, nt, src, inject_mode=O.INJECT_ACCUMULATE)
+    tr, F = O.run(p, z, lam, lam, w, st)
+    acc = np.float32(0)
+    exp = []
+    d2 = np.float32(np.float32(dt) * np.float32(dt))
+    for it in range(nt):
+        acc = np.float32(acc + np.float32(d2 * w[it]))
+        exp.append(acc if where == "interior" else np.float32(0))
+    for c in (3, 4, 5):
+        assert np.array_equal(tr[:, 0, c], np.array(exp, np.float32))
+    assert not F[:3].any()
+    p.inject_mode = O.INJECT_DOUBLE
+    tr, F = O.run(p, z, lam, lam, w, st)
+    assert not tr.any() and not F.any()
