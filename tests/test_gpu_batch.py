@@ -228,3 +228,36 @@ def test_batched_accumulate_sources_match_single_shots(sgfd_mod):
             b.close()
         for s in solvers:
             s.close()
+
+
+@pytest.mark.parametrize("seed", [400, 401, 402])
+def test_random_batches_match_single_shots(sgfd_mod, seed):
+    """Three seeded random shots in one batch with another seeded geometry's
+    box shapes, a persistent or a plain grid, graph or eager, stations
+    decimated by 1..3: bit-identical to the shots run alone."""
+    from conftest import random_case, random_workload
+
+    sgfd = sgfd_mod
+    rng = np.random.default_rng(seed)
+    wls = [random_workload(seed * 10 + i, nt=30)[0] for i in range(3)]
+    for wl in wls:
+        wl.interval = int(rng.integers(1, 4))
+    tile = random_case(seed)[2]
+    grid, graph = int(rng.choice([0, 64, 300])), bool(seed % 2)
+    _say(f"random batch {seed}: box shapes {tile}, grid {grid}, graph {graph}")
+    ref = [_run(wl) for wl in wls]
+    solvers = [wl.solver() for wl in wls]
+    b = None
+    try:
+        b = sgfd.Batch(solvers)
+        b.set_launch(*tile, grid=grid, graph=graph)
+        b.step(30)
+        b.synchronize()
+        for s, (tr0, F0) in zip(solvers, ref):
+            assert np.array_equal(s.fields(), F0)
+            assert np.array_equal(s.stations(), tr0)
+    finally:
+        if b is not None:
+            b.close()
+        for s in solvers:
+            s.close()

@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_golden
+from conftest import GOLDEN, load_golden, random_case as _random_case
 
 pytestmark = pytest.mark.gpu
 
@@ -339,25 +339,6 @@ def test_medium_from_velocity_matches_medium(sgfd_mod):
         finally:
             s.close()
     assert np.abs(out[0]).max() > 0 and np.array_equal(out[0], out[1])
-
-
-def _random_case(seed):
-    """A seeded random geometry: extents 11 + 2P .. 11 + 2P + 90, P in
-    0..12, one of the five box shapes for the interior and the PML boxes, a
-    z-chunk of 1..40 planes, the source anywhere in the updated range and
-    three stations within 6 cells of it.  In 30 steps the wave travels about
-    7 cells; a station much farther out records only the stencil's numerical
-    precursor (1e-30 and below, near the bottom of the float range), whose
-    relative error carries no meaning."""
-    rng = np.random.default_rng(seed)
-    pml = int(rng.integers(0, 13))
-    dims = [int(rng.integers(11 + 2 * pml, 11 + 2 * pml + 91)) for _ in range(3)]
-    shapes = [(4, 1), (8, 1), (8, 2), (16, 1), (16, 2)]
-    a, b = (shapes[int(i)] for i in rng.integers(0, 5, 2))
-    tile = (a[0], a[1], b[0], b[1], int(rng.integers(1, 41)))
-    src = tuple(int(rng.integers(5, n - 5)) for n in dims)
-    near = lambda: [min(max(c + int(rng.integers(-6, 7)), 5), n - 6) for c, n in zip(src, dims)]
-    return dims, pml, tile, src, np.array([near() for _ in range(3)], np.int32)
 
 
 RANDOM_SEEDS = range(24)

@@ -124,3 +124,37 @@ def test_compensation_rejected_where_it_does_not_apply(setup):
         s.set_compensation("all")
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("seed", [300, 301, 302, 303])
+def test_16bit_random_geometries(seed):
+    """Seeded random geometries and box shapes: FP16 and BF16 storage give
+    the same bits with the default and the random box shape, and stay within
+    their error levels of the FP64 build (fields rel-L2 2e-2 / 1.5e-1)."""
+    import sgfd
+    from conftest import random_workload
+
+    sgfd.lib()
+    wl, tile = random_workload(seed)
+    s = wl.solver("fp64")
+    try:
+        s.step(wl.nt)
+        F64 = s.fields(np.float64)
+    finally:
+        s.close()
+    sc = wl.pilot_scales(steps=wl.nt)
+    for prec, lim in (("fp16", 2e-2), ("bf16", 1.5e-1)):
+        out = []
+        for t in (None, tile):
+            s = wl.solver(prec, scales=sc)
+            try:
+                if t:
+                    s.set_tile(*t)
+                s.step(wl.nt)
+                out.append((s.stations(), s.fields(np.float64)))
+            finally:
+                s.close()
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), prec
+        err = float(np.linalg.norm(out[0][1] - F64) / np.linalg.norm(F64))
+        _say(f"seed {seed} {prec}: fields rel-L2 {err:.2e} against FP64")
+        assert np.isfinite(out[0][1]).all() and err < lim, prec

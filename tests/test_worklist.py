@@ -56,8 +56,24 @@ def test_random_geometries(checker):
 
 def test_gpu_random_parity_geometries(checker):
     """The geometries and box shapes of the GPU random-geometry parity test."""
-    from test_gpu_parity import RANDOM_SEEDS, _random_case
+    from conftest import random_case
+    from test_gpu_parity import RANDOM_SEEDS
 
     for seed in RANDOM_SEEDS:
-        (nx, ny, nz), pml, tile, _, _ = _random_case(seed)
+        (nx, ny, nz), pml, tile, _, _ = random_case(seed)
         _check(checker, nx, ny, nz, pml, tile[0], tile[2], tile[4])
+
+
+def test_gpu_random_16bit_and_batch_geometries(checker):
+    """The geometries and box shapes of the GPU random 16-bit and batch tests
+    (a batch applies one geometry's box shapes to three others)."""
+    from conftest import random_case
+
+    for seed in (300, 301, 302, 303):
+        (nx, ny, nz), pml, tile, _, _ = random_case(seed)
+        _check(checker, nx, ny, nz, pml, tile[0], tile[2], tile[4])
+    for seed in (400, 401, 402):
+        tile = random_case(seed)[2]
+        for i in range(3):
+            (nx, ny, nz), pml, _, _, _ = random_case(seed * 10 + i)
+            _check(checker, nx, ny, nz, pml, tile[0], tile[2], tile[4])
