@@ -64,8 +64,17 @@ def signed_index(n):
     return np.where(m <= (n - 1) // 2, m, m - n).astype(np.float64)
 
 
+def kind_id(kind):
+    """Spectrum kind by name ("reference", "gaussian", "von_karman") or id."""
+    k = KINDS.get(kind) if isinstance(kind, str) else kind
+    if k not in KINDS.values():
+        raise ValueError(f"unknown spectrum kind {kind!r}")
+    return k
+
+
 def spectrum(nx, ny, nz, kind, a=(10.0, 10.0, 10.0), order=10.0, l=50.0):
-    """P on the (nz, ny, nx) bin grid."""
+    """P on the (nz, ny, nx) bin grid; kind by name or id."""
+    kind = kind_id(kind)
     sx, sy, sz = signed_index(nx), signed_index(ny), signed_index(nz)
     if kind == KIND_REFERENCE:
         kx, ky, kz = sx / l, sy / l, sz / l
@@ -94,7 +103,7 @@ def raw_field(nx, ny, nz, kind, a=(10.0, 10.0, 10.0), order=10.0, l=50.0, seed=0
     """The field before normalisation: |ifftn(psi)| (reference) or Re (others),
     unnormalised like hipFFT's backward transform; FP64."""
     f = np.fft.ifftn(psi_grid(nx, ny, nz, kind, a, order, l, seed)) * (nx * ny * nz)
-    return np.abs(f) if kind == KIND_REFERENCE else f.real
+    return np.abs(f) if kind_id(kind) == KIND_REFERENCE else f.real
 
 
 def normalise(x, eps=0.1, v0=2000.0):
@@ -106,7 +115,7 @@ def normalise(x, eps=0.1, v0=2000.0):
 def random_medium(nx, ny, nz, kind="reference", a=(10.0, 10.0, 10.0), order=10.0, l=50.0, seed=0, v0=2000.0,
                   eps=0.1):
     """FP64 result, shape (nz, ny, nx)."""
-    return normalise(raw_field(nx, ny, nz, KINDS[kind], a, order, l, seed), eps, v0)
+    return normalise(raw_field(nx, ny, nz, kind, a, order, l, seed), eps, v0)
 
 
 def brute_inverse_dft(psi):

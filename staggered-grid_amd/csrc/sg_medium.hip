@@ -202,10 +202,18 @@ int random_medium(int nx, int ny, int nz, const sg_spectrum *spec, unsigned long
     }
     if ((e = hipMemcpy(tab, htab.data(), sizeof(double) * htab.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "table copy");
-    // the plan (and any kernel set-up of the FFT library) before the timed part
-    if (hipfftPlan3d(&plan, nz, ny, nx, HIPFFT_C2C) != HIPFFT_SUCCESS) {
+    // the plan (and any kernel set-up of the FFT library) before the timed
+    // part; axes of one point are left out (a 3-D transform with unit axes is
+    // the lower-rank transform of the others, in the same memory order)
+    int dims[3], rank = 0;
+    for (int d : {nz, ny, nx})
+        if (d > 1) dims[rank++] = d;
+    hipfftResult pr = rank == 3   ? hipfftPlan3d(&plan, dims[0], dims[1], dims[2], HIPFFT_C2C)
+                      : rank == 2 ? hipfftPlan2d(&plan, dims[0], dims[1], HIPFFT_C2C)
+                                  : hipfftPlan1d(&plan, dims[0], HIPFFT_C2C, 1);
+    if (pr != HIPFFT_SUCCESS) {
         cleanup();
-        return fail(SG_ERR_HIP, "sg_random_medium: hipfftPlan3d failed");
+        return fail(SG_ERR_HIP, "sg_random_medium: FFT plan creation failed");
     }
     have_plan = true;
     if (hipfftSetStream(plan, st) != HIPFFT_SUCCESS) {
@@ -238,9 +246,14 @@ int random_medium(int nx, int ny, int nz, const sg_spectrum *spec, unsigned long
     }
     const double mean = s1 / (double)n, var = s2 / (double)n - mean * mean;
     const double mom[2] = {mean, var > 0 ? std::sqrt(var) : 0.0};
-    if (!(mom[1] > 0)) {
+    // The complex64 field holds each value to ~6e-8 of the mean, so the
+    // normalised field is accurate to ~6e-8 / (std / |mean|).  A spread
+    // below 1e-4 of the mean (errors above ~6e-4 of eps) is refused rather
+    // than scaled up to eps.
+    if (!(mom[1] > 1e-4 * std::fabs(mom[0]))) {
         cleanup();
-        return fail(SG_ERR_INVALID, "sg_random_medium: the field has zero variance (spectrum too narrow for the grid)");
+        return fail(SG_ERR_INVALID, "sg_random_medium: the field's spread is below 1e-4 of its mean, beyond single "
+                                    "precision (spectrum too narrow for the grid)");
     }
     hipLaunchKernelGGL(sg::k_finish, dim3(grid), dim3(sg::MTHREADS), 0, st, psi, n, take_abs, mom[0], eps / mom[1],
                        v0, dout);

@@ -58,17 +58,42 @@ def test_random_medium_matches_oracle(sgfd_mod, c):
     np.testing.assert_allclose(v, 2000.0 * (1 + want), rtol=2e-6)
 
 
-def test_random_medium_full_size_properties(sgfd_mod):
-    nx, ny, nz = 512, 512, 384
-    _say("random medium 512x512x384, reference spectrum")
-    d = sgfd_mod.random_medium(nx, ny, nz, "reference", seed=9, v0=0.0, eps=0.1)
-    d64 = d.astype(np.float64)
-    assert abs(d64.mean()) < 1e-7
-    assert abs(np.sqrt(np.mean(d64 * d64)) - 0.1) < 1e-7
-    again = sgfd_mod.random_medium(nx, ny, nz, "reference", seed=9, v0=0.0, eps=0.1)
-    assert np.array_equal(d, again)
-    other = sgfd_mod.random_medium(nx, ny, nz, "reference", seed=10, v0=0.0, eps=0.1)
-    assert not np.array_equal(d, other)
+
+SMALL = [
+    dict(nx=7, ny=3, nz=1, kind="reference", a=(4.7, 4.5, 5.3), order=8.2, l=7.3, seed=11),
+    dict(nx=6, ny=1, nz=2, kind="von_karman", a=(4.9, 5.3, 2.3), order=0.35, l=45.8, seed=12),
+    dict(nx=32, ny=2, nz=3, kind="gaussian", a=(11.7, 3.3, 7.9), order=1.75, l=29.0, seed=13),
+    dict(nx=2, ny=2, nz=1, kind="reference", a=(10.0, 2.7, 8.1), order=8.8, l=17.0, seed=14),
+    # the two fastest axes of one point
+    dict(nx=1, ny=1, nz=3, kind="gaussian", a=(1.0, 1.0, 0.8), order=1.0, l=1.0, seed=17),
+    dict(nx=2, ny=1, nz=5, kind="gaussian", a=(0.9, 1.0, 1.1), order=1.0, l=1.0, seed=18),
+    dict(nx=1, ny=5, nz=4, kind="von_karman", a=(3.0, 2.0, 4.0), order=0.5, l=1.0, seed=15),
+    dict(nx=1, ny=1, nz=40, kind="reference", a=(3.0, 3.0, 3.0), order=2.0, l=6.0, seed=16),
+]
+
+
+@pytest.mark.parametrize("c", SMALL, ids=[f"{c['kind']}-{c['nx']}x{c['ny']}x{c['nz']}" for c in SMALL])
+def test_random_medium_degenerate_axes(sgfd_mod, c):
+    """Axes of one to three points: the spectral method still matches the
+    oracle (delta rel-L2 <= 1e-5)."""
+    args = {k: c[k] for k in ("kind", "a", "order", "l", "seed")}
+    d = sgfd_mod.random_medium(c["nx"], c["ny"], c["nz"], v0=0.0, eps=0.1, **args)
+    want = M.random_medium(c["nx"], c["ny"], c["nz"], v0=0.0, eps=0.1, **args)
+    assert _rel(d, want) <= 1e-5
+
+
+@pytest.mark.parametrize("a,seed", [((5.7, 11.5, 11.3), 751968576823), ((2.70, 2.75, 5.14), 777253584999)])
+def test_random_medium_refuses_a_flat_field(sgfd_mod, a, seed):
+    """Correlation lengths far beyond a 1x1x3 grid (Gaussian spectrum): the
+    non-constant modes are 0 or 5e-7 of the constant one, the FP64 field's
+    spread is 0 or 9e-7 of its mean.  Single precision holds each value to
+    ~6e-8 of the mean, so the library refuses (spread below 1e-4 of the mean)
+    instead of scaling rounding noise up to eps; the oracle divides by the
+    spread there."""
+    raw = M.raw_field(1, 1, 3, "gaussian", a=a, order=7.8, l=47.4, seed=seed)
+    assert raw.std() < 1e-6 * abs(raw.mean())
+    with pytest.raises(sgfd_mod.SgError, match="spread"):
+        sgfd_mod.random_medium(1, 1, 3, "gaussian", a=a, order=7.8, l=47.4, seed=seed, v0=0.0, eps=0.1)
 
 
 def test_random_medium_feeds_the_solver(sgfd_mod):

@@ -123,3 +123,11 @@ def test_random_medium_rejects_bad_arguments():
         args.update(kw)
         with pytest.raises(sgfd.SgError):
             sgfd.random_medium(8, 8, 8, **args)
+
+
+def test_spectrum_kind_by_name_or_id():
+    for name, kid in M.KINDS.items():
+        assert np.array_equal(M.spectrum(5, 4, 3, name, (2.0, 3.0, 1.5), 0.7, 9.0),
+                              M.spectrum(5, 4, 3, kid, (2.0, 3.0, 1.5), 0.7, 9.0))
+    with pytest.raises(ValueError):
+        M.spectrum(5, 4, 3, "gauss")
