@@ -65,11 +65,12 @@ def test_gpu_random_parity_geometries(checker):
 
 
 def test_gpu_random_16bit_and_batch_geometries(checker):
-    """The geometries and box shapes of the GPU random 16-bit and batch tests
-    (a batch applies one geometry's box shapes to three others)."""
+    """The geometries and box shapes of the GPU random 16-bit, per-cell
+    damping and batch tests (a batch applies one geometry's box shapes to
+    three others)."""
     from conftest import random_case
 
-    for seed in (300, 301, 302, 303):
+    for seed in (300, 301, 302, 303, 500, 501, 502):
         (nx, ny, nz), pml, tile, _, _ = random_case(seed)
         _check(checker, nx, ny, nz, pml, tile[0], tile[2], tile[4])
     for seed in (400, 401, 402):
