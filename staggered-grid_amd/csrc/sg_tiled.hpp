@@ -1031,11 +1031,19 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 // ---------------------------------------------------------------------------
 // kernels: a grid-stride loop over a work-list (one or many shots)
 // ---------------------------------------------------------------------------
-// Register caps (minimum waves per SIMD) of the FP32 face instantiations
-// (DM 1, 2, 4: SG_PML_WAVES) and of the interior ones (SG_INT_WAVES; 16-bit
-// storage: SG_INT_WAVES16); edges and corners are never capped
+// Register caps (minimum waves per SIMD) of the face instantiations (DM 1,
+// 2, 4: SG_PML_WAVES; 16-bit storage: SG_PML_WAVES16) and of the interior
+// ones (SG_INT_WAVES; 16-bit storage: SG_INT_WAVES16); edges and corners
+// are never capped
 #ifndef SG_PML_WAVES
 #define SG_PML_WAVES 1
+#endif
+// 16-bit storage, measured at 512x512x384 and 500^3 (FP16 31.3 -> 32.0 and
+// 37.7 -> 39.4 Gpt/s, BF16 30.5 -> 31.5; FP32 unaffected): faces at 3 waves
+// (the x-face kernels need 162-166 VGPRs then, 169-174 uncapped, no spill),
+// the interior velocity sweep at 4 (128 VGPRs, a 12-byte spill)
+#ifndef SG_PML_WAVES16
+#define SG_PML_WAVES16 3
 #endif
 #ifndef SG_INT_WAVES
 #define SG_INT_WAVES 1
@@ -1043,16 +1051,23 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #ifndef SG_INT_WAVES16
 #define SG_INT_WAVES16 1
 #endif
-template <class T, int DM, int NY>
+#ifndef SG_INT_WAVES16_V
+#define SG_INT_WAVES16_V 4
+#endif
+// W: 0 velocity, 1 stress (the 16-bit interior velocity sweep has its own
+// cap); the compensated 16-bit kernels (CP) are never capped (they would
+// spill: 92-120 bytes per lane at 4 waves)
+template <class T, int DM, int NY, int W, bool CP>
 struct WavesCap {
-    static constexpr bool F32 = sizeof(T) == 4;
+    static constexpr bool F32 = sizeof(T) == 4, H = sizeof(T) == 2 && !CP;
+    static constexpr int i16 = W == 0 ? SG_INT_WAVES16_V : SG_INT_WAVES16;
     static constexpr int v = DM == 7 ? 1
-                             : DM   ? (F32 ? SG_PML_WAVES : 1)
-                                    : (NY == 1 ? (F32 ? SG_INT_WAVES : (sizeof(T) == 2 ? SG_INT_WAVES16 : 1)) : 1);
+                             : DM   ? (F32 ? SG_PML_WAVES : (H ? SG_PML_WAVES16 : 1))
+                                    : (NY == 1 ? (F32 ? SG_INT_WAVES : (H ? i16 : 1)) : 1);
 };
 
 template <class T, class C, int TY, int NY, int DM, bool CP>
-__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, DM, NY>::v))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, DM, NY, 0, CP>::v))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
                                                             const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ VelLds<C, TY> L;
@@ -1064,7 +1079,7 @@ __global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(Wav
 }
 
 template <class T, class C, int TY, int NY, int DM, bool CP>
-__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, DM, NY>::v))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, DM, NY, 1, CP>::v))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
                                                           const WorkItem *__restrict__ items, int nitems)
 {
     __shared__ StrLds<C, TY> L;
