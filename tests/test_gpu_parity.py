@@ -226,6 +226,36 @@ def test_reference_cuda_station_file(sgfd_mod):
     assert _rel(data[:n], ref[:n]) < 1e-4
 
 
+
+def test_propagate_per_step_matches_run(sgfd_mod):
+    """The reference's per-step entry, propagate(timeIndex, sourceValue)
+    (..._concise/ElasticWaveEquation.cu:194-206), driven with
+    sourceFunction(it) for every step, gives the same bits as run(); a
+    time index out of order is refused."""
+    sgfd = sgfd_mod
+    nx, ny, nz, pml, nt = 64, 56, 48, 6, 80
+    shape = (nz, ny, nx)
+    med = (np.full(shape, np.float32(1 / 2600.0), np.float32), np.full(shape, np.float32(2600.0 * 4.0e6), np.float32),
+           np.full(shape, np.float32(2600.0 * 4.0e6 / 3.0), np.float32))
+    args = (nt, None, pml, (nx, ny, nz), (5e-4, 10.0, 10.0, 10.0), med, (32, 28, 24), [(40, 28, 24), (20, 30, 30)],
+            sgfd.diff_coef("reference"), 25.0)
+    a = sgfd.ElasticWaveEquation(*args)
+    try:
+        ref = a.run(snap_every=0)
+        Fa = a.solver.fields()
+    finally:
+        a.close()
+    b = sgfd.ElasticWaveEquation(*args)
+    try:
+        with pytest.raises(ValueError):
+            b.propagate(1, 0.0)
+        for it in range(nt):
+            b.propagate(it, b.sourceFunction(it))
+        assert np.array_equal(b.solver.stations(), ref) and np.array_equal(b.solver.fields(), Fa)
+        assert np.abs(ref).max() > 0
+    finally:
+        b.close()
+
 def test_reference_cuda_station_file_all_rows(sgfd_mod):
     """All 1500 rows of the reference's CUDA station file, with the CUDA
     path's accumulating source AND its per-cell damping (tau 4.3, local S
