@@ -1038,6 +1038,11 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #ifndef SG_PML_WAVES
 #define SG_PML_WAVES 1
 #endif
+// FP32 velocity faces at 3 waves (x face 170 -> 168 VGPRs): 512x512x384
+// 27.70 -> 27.79, 256^3 20.55 -> 20.91 Gpt/s; the stress faces lose with it
+#ifndef SG_PML_WAVES_V
+#define SG_PML_WAVES_V 3
+#endif
 // 16-bit storage, measured at 512x512x384 and 500^3 (FP16 31.3 -> 32.0 and
 // 37.7 -> 39.4 Gpt/s, BF16 30.5 -> 31.5; FP32 unaffected): faces at 3 waves
 // (the x-face kernels need 162-166 VGPRs then, 169-174 uncapped, no spill),
@@ -1061,8 +1066,9 @@ template <class T, int DM, int NY, int W, bool CP>
 struct WavesCap {
     static constexpr bool F32 = sizeof(T) == 4, H = sizeof(T) == 2 && !CP;
     static constexpr int i16 = W == 0 ? SG_INT_WAVES16_V : SG_INT_WAVES16;
+    static constexpr int f32 = W == 0 ? SG_PML_WAVES_V : SG_PML_WAVES;
     static constexpr int v = DM == 7 ? 1
-                             : DM   ? (F32 ? SG_PML_WAVES : (H ? SG_PML_WAVES16 : 1))
+                             : DM   ? (F32 ? f32 : (H ? SG_PML_WAVES16 : 1))
                                     : (NY == 1 ? (F32 ? SG_INT_WAVES : (H ? i16 : 1)) : 1);
 };
 
