@@ -227,10 +227,19 @@ def precision_study(args):
     K = wl.nt
     res = {}
     ref = None
-    for prec in ("fp64", "fp32", "fp16", "bf16"):
-        log(f"bench: {prec}, {K} steps")
+    # (name, storage, rounding, compensated fields)
+    modes = [("fp64", "fp64", None, None), ("fp32", "fp32", None, None), ("fp16", "fp16", None, None),
+             ("bf16", "bf16", None, None), ("fp16-stochastic", "fp16", "stochastic", None),
+             ("fp16-compensated", "fp16", None, "all"), ("bf16-stochastic", "bf16", "stochastic", None),
+             ("bf16-compensated", "bf16", None, "all")]
+    for name, prec, rounding, comp in modes:
+        log(f"bench: {name}, {K} steps")
         s = wl.solver(precision=prec)
         try:
+            if rounding:
+                s.set_rounding(rounding)
+            if comp:
+                s.set_compensation(comp)
             _, _, ms = s.step_timed(K)
             tr = s.stations().astype(np.float64)
         finally:
@@ -238,7 +247,7 @@ def precision_study(args):
         if ref is None:
             ref = tr
         err = float(np.linalg.norm(tr - ref) / np.linalg.norm(ref))
-        res[prec] = {"gpt_per_s": wl.points * K / (ms / 1e3) / 1e9, "ms_per_step": ms / K,
+        res[name] = {"gpt_per_s": wl.points * K / (ms / 1e3) / 1e9, "ms_per_step": ms / K,
                      "seismogram_rel_l2_vs_fp64": err}
     ratio = res["fp16"]["gpt_per_s"] / res["fp32"]["gpt_per_s"]
     out = {"metric": "FP16-store vs FP32 at fixed seismogram error", "value": ratio, "unit": "x (FP16/FP32 Gpt/s)",
@@ -268,7 +277,8 @@ def main():
     ap.add_argument("--compare-sequential", action="store_true",
                     help="batched32: also time the shots one solver at a time")
     ap.add_argument("--precision-study", action="store_true",
-                    help="FP64/FP32/FP16/BF16 over the full workload: speed and seismogram error against FP64")
+                    help="every storage precision and 16-bit mode over the full workload: speed and seismogram "
+                         "error against FP64")
     args = ap.parse_args()
 
     if args.gpus != 1:
