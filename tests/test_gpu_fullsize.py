@@ -3,7 +3,9 @@
 * 256^3 random medium (P16): the first 40 steps against the C oracle (its
   OpenMP build finishes them in seconds): every wavefield rel-L2 <= 1e-5.
 * 512x512x384 four-layer model (P20), the full 2000 steps: FP32 station
-  traces against the FP64 build of the same kernels, rel-L2 <= 1e-5.
+  traces against the FP64 build of the same kernels, rel-L2 <= 1e-5; FP16
+  and BF16 storage (scales from an FP32 pilot run) within 1e-2 and 8e-2,
+  their measured levels being 7.6e-3 and 6.2e-2.
 * 1024x1024x640 four-layer model (P24), the full 1500 steps: FP32 against
   the FP64 build on the 256x256 surface receivers and on a seeded 2 %
   subsample of the points of vz and sxx, rel-L2 <= 1e-5.
@@ -74,11 +76,11 @@ def test_random256_first_steps_against_oracle(mods, oracle_mod):
         assert e < 1e-5, (sgfd.FIELDS[f], e)
 
 
-def test_layered512_fp32_against_fp64_full_run(mods):
+def test_layered512_precisions_against_fp64_full_run(mods):
     sgfd, configs = mods
     wl = configs.layered512()
     res = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp16", "bf16"):
         _say(f"layered512 2000 steps {prec}")
         s = wl.solver(prec)
         try:
@@ -87,9 +89,12 @@ def test_layered512_fp32_against_fp64_full_run(mods):
             res[prec] = s.stations().astype(np.float64)
         finally:
             s.close()
-    e = _rel(res["fp32"], res["fp64"])
-    _say(f"seismogram rel-L2 fp32 vs fp64: {e:.3e}")
-    assert e < 1e-5
+    err = {p: _rel(res[p], res["fp64"]) for p in ("fp32", "fp16", "bf16")}
+    _say("seismogram rel-L2 against fp64: " + ", ".join(f"{p} {e:.3e}" for p, e in err.items()))
+    assert err["fp32"] < 1e-5
+    # 16-bit storage at its measured levels (7.6e-3 and 6.2e-2) with margin
+    assert 1e-3 < err["fp16"] < 1e-2
+    assert 1e-2 < err["bf16"] < 8e-2
 
 
 def test_large1024_fp32_against_fp64_subsample(mods):
