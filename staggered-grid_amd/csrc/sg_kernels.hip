@@ -84,7 +84,7 @@ template <class T, class C>
 __device__ __forceinline__ T store_val(const Shot<T, C> &s, int f, long long o, C v)
 {
     if (sizeof(T) == 2 && s.round_mode)
-        return CvtSR<T, C>::st(v, s.fscale[f], round_key((uint32_t)o, (uint32_t)*s.d_step, (uint32_t)f));
+        return CvtSR<T, C>::st(v, s.fscale[f], sr_bits((uint32_t)o, (uint32_t)*s.d_step, (uint32_t)f));
     return Cvt<T, C>::st(v, s.fscale[f]);
 }
 

@@ -69,9 +69,19 @@ __host__ __device__ __forceinline__ uint32_t round_key(uint32_t point, uint32_t 
     return hash32(point * 0x9e3779b9u ^ hash32(step * 16u + field));
 }
 
+// 16 random bits for storing field f at (point, step): fields 2m and 2m+1
+// take the low and the high half of one hash, so a point costs 6 hashes
+// instead of 9 (the stochastic-rounding overhead was mostly hashing).
+__host__ __device__ __forceinline__ uint32_t sr_bits(uint32_t point, uint32_t step, uint32_t field)
+{
+    const uint32_t k = round_key(point, step, field >> 1);
+    return (field & 1u) ? k >> 16 : k & 0xffffu;
+}
+
 // Stochastic rounding of a float to FP16: the nearer neighbour x_n (round to
-// nearest even) or the farther one x_f, with P(x_f) = |x - x_n| / |x_f - x_n|,
-// so the stored value is unbiased.  r: 32 random bits.
+// nearest even) or the farther one x_f, with P(x_f) = |x - x_n| / |x_f - x_n|
+// (resolved to 2^-16), so the stored value is unbiased.  r: random bits, the
+// low 16 used.
 __host__ __device__ __forceinline__ _Float16 half_sr(float x, uint32_t r)
 {
     const _Float16 h = (_Float16)x;
@@ -84,7 +94,7 @@ __host__ __device__ __forceinline__ _Float16 half_sr(float x, uint32_t r)
     else fb = (uint16_t)(((d > 0.0f) == (xh > 0.0f)) ? hb + 1 : hb - 1);     // one step away from / towards zero
     const _Float16 f = __builtin_bit_cast(_Float16, fb);
     const float p = d / ((float)f - xh);                                        // in (0, 1/2]
-    return ((float)(r >> 8) * (1.0f / 16777216.0f) < p) ? f : h;
+    return ((float)(r & 0xffffu) * (1.0f / 65536.0f) < p) ? f : h;
 }
 
 // Stochastic rounding to BF16: add 16 random bits below the kept mantissa

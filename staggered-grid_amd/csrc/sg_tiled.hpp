@@ -595,7 +595,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                     const uint32_t pt = t.ro[r] / (unsigned)sizeof(T) + (uint32_t)k * (uint32_t)sz;
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
-                        bst_nt(pv[c], t.ro[r], CvtSR<T, C>::st(vn[c], s.fscale[VX + c], round_key(pt, step, VX + c)));
+                        bst_nt(pv[c], t.ro[r], CvtSR<T, C>::st(vn[c], s.fscale[VX + c], sr_bits(pt, step, VX + c)));
                 } else {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) bst_nt(pv[c], t.ro[r], Cvt<T, C>::st(vn[c], s.fscale[VX + c]));
@@ -974,7 +974,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                     const uint32_t pt = t.ro[r] / (unsigned)sizeof(T) + (uint32_t)k * (uint32_t)sz;
 #pragma unroll
                     for (int c = 0; c < 6; ++c)
-                        bst_nt(ps[c], t.ro[r], CvtSR<T, C>::st(nw[c], s.fscale[SXX + c], round_key(pt, step, SXX + c)));
+                        bst_nt(ps[c], t.ro[r], CvtSR<T, C>::st(nw[c], s.fscale[SXX + c], sr_bits(pt, step, SXX + c)));
                 } else {
 #pragma unroll
                     for (int c = 0; c < 6; ++c) bst_nt(ps[c], t.ro[r], Cvt<T, C>::st(nw[c], s.fscale[SXX + c]));
