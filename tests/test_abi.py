@@ -117,3 +117,30 @@ def test_cpu_class_dropin_header_compiles_and_links(sgfd_mod, tmp_path):
                     os.path.join(ROOT, "staggered-grid_amd", "examples", "main_cpu.cpp"),
                     "-L" + lib_dir, "-lsgfd", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
     assert exe.exists()
+
+
+C_PROGRAM = r"""#include "sgfd.h"
+#include <stdio.h>
+int main(void)
+{
+    float c[5];
+    int rc = sg_diff_coef(1, c);
+    printf("%d %d %.6f\n", sg_api_version(), rc, c[0]);
+    return rc;
+}
+"""
+
+
+def test_c_header_is_plain_c99(tmp_path):
+    """include/sgfd.h is the boundary an FFI binds: it compiles as C99 with
+    -pedantic, and a C program links against the library."""
+    import subprocess
+
+    src = tmp_path / "abi.c"
+    src.write_text(C_PROGRAM)
+    lib_dir = os.path.join(ROOT, "staggered-grid_amd", "sgfd")
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    str(src), "-L" + lib_dir, "-lsgfd", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=60).stdout.split()
+    assert out[0] == "1" and out[1] == "0" and abs(float(out[2]) - 19845 / 16384) < 1e-6
