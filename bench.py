@@ -272,6 +272,7 @@ def main():
                     help="interior box rows and rows per thread, PML box rows and rows per thread, "
                          "z planes per box (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage leg of an FP32 run")
     ap.add_argument("--streams", type=int, default=None, help="streams per sweep (1, 2, 3)")
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
     ap.add_argument("--compare-sequential", action="store_true",
@@ -307,33 +308,18 @@ def main():
     t = time.perf_counter()
     wl = configs.BUILDERS[args.workload]()
     log(f"bench: workload ready in {time.perf_counter() - t:.1f} s, courant {wl.courant():.3f}")
-    s = wl.solver(precision=args.precision, kernel=args.kernel, time_length=args.warmup + args.steps)
-    if args.tile:
-        s.set_tile(*args.tile)
-    if args.streams:
-        s.set_streams(args.streams)
-    log(f"bench: warmup {args.warmup} steps")
-    s.step(args.warmup)
-    s.synchronize()
-    log(f"bench: timing {args.steps} steps")
-    ms_v, ms_s, ms_tot = s.step_timed(args.steps)
-    s.synchronize()
-    vbytes, sbytes = s.sweep_bytes()
-    energy = s.energy()
-    s.close()
-
     K = args.steps
-    sec = ms_tot / 1e3
+    r = timed_run(sgfd, wl, args, args.precision)
+    sec = r["ms_total"] / 1e3
     value = wl.points * K / sec / 1e9
     # updated points (5-cell halo excluded) and the undamped interior among them
     upd = (wl.nx - 10) * (wl.ny - 10) * (wl.nz - 10)
     inner = max(wl.nx - 10 - 2 * wl.pml, 0) * max(wl.ny - 10 - 2 * wl.pml, 0) * max(wl.nz - 10 - 2 * wl.pml, 0)
     # the roofline also as a fraction of this device's measured copy bandwidth
     copy_gbps = sgfd.copy_bandwidth(1 << 30, 20)
-    avg_v, avg_s = ms_v / K, ms_s / K
-    dom = ("stress", avg_s, sbytes) if avg_s >= avg_v else ("velocity", avg_v, vbytes)
-    achieved = dom[2] / (dom[1] * 1e-3) / 1e9
-    traffic = read_traffic(args.workload, args.precision)
+    roof = r["roofline"]
+    roof["copy_gbps"] = copy_gbps
+    roof["copy_frac"] = roof["achieved"] / copy_gbps
     out = {
         "metric": METRIC,
         "value": value,
@@ -341,23 +327,29 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": ms_tot / K,
+        "ms_per_step": r["ms_total"] / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": "synthetic",
         "config": {"workload": wl.description, "grid": [wl.nx, wl.ny, wl.nz], "pml": wl.pml,
                    "points": {"total": wl.points, "updated": upd, "interior": inner},
                    "precision": args.precision, "kernel": args.kernel, "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "alg_bytes_per_launch": dom[2], "avg_launch_ms": dom[1],
-                     "copy_gbps": copy_gbps, "copy_frac": achieved / copy_gbps},
-        "sweeps_ms": {"velocity_avg": avg_v, "stress_avg": avg_s},
-        "step_hbm_frac": (vbytes + sbytes) * K / sec / 1e9 / HBM_PEAK_GBPS,
-        "energy_after": list(energy),
+        "roofline": roof,
+        "sweeps_ms": r["sweeps_ms"],
+        "step_hbm_frac": r["step_hbm_frac"],
+        "energy_after": r["energy"],
     }
+    if args.precision == "fp32" and not args.no_fp16:
+        # the metric's second half: the same workload with FP16 storage
+        # (scales from an FP32 pilot run), timed the same way
+        r16 = timed_run(sgfd, wl, args, "fp16")
+        v16 = wl.points * K / (r16["ms_total"] / 1e3) / 1e9
+        out["fp16"] = {"value": v16, "unit": "Gpt-updates/s", "ratio_to_fp32": v16 / value,
+                       "ms_per_step": r16["ms_total"] / K, "dtype": DTYPES["fp16"], "roofline": r16["roofline"],
+                       "sweeps_ms": r16["sweeps_ms"], "target_ratio": 1.7,
+                       "seismogram_rel_l2_vs_fp64": read_precision_error(args.workload, "fp16")}
     if not args.no_cpu:
         log("bench: CPU baseline (reference CPU solver, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(wl)
@@ -366,6 +358,52 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+DTYPES = {"fp32": "f32", "fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}
+
+
+def timed_run(sgfd, wl, args, precision):
+    """args.warmup untimed steps, then args.steps steps with HIP events
+    around each sweep (sg_step_timed); the dominant sweep's roofline."""
+    s = wl.solver(precision=precision, kernel=args.kernel, time_length=args.warmup + args.steps)
+    try:
+        if args.tile:
+            s.set_tile(*args.tile)
+        if args.streams:
+            s.set_streams(args.streams)
+        log(f"bench: {precision} warmup {args.warmup} steps")
+        s.step(args.warmup)
+        s.synchronize()
+        log(f"bench: {precision} timing {args.steps} steps")
+        ms_v, ms_s, ms_tot = s.step_timed(args.steps)
+        s.synchronize()
+        vbytes, sbytes = s.sweep_bytes()
+        energy = s.energy()
+    finally:
+        s.close()
+    K = args.steps
+    avg_v, avg_s = ms_v / K, ms_s / K
+    dom = ("stress", avg_s, sbytes) if avg_s >= avg_v else ("velocity", avg_v, vbytes)
+    achieved = dom[2] / (dom[1] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": read_traffic(args.workload, precision),
+            "alg_bytes_per_launch": dom[2], "avg_launch_ms": dom[1]}
+    return {"ms_total": ms_tot, "roofline": roof, "sweeps_ms": {"velocity_avg": avg_v, "stress_avg": avg_s},
+            "step_hbm_frac": (vbytes + sbytes) * K / (ms_tot / 1e3) / 1e9 / HBM_PEAK_GBPS, "energy": list(energy)}
+
+
+def read_precision_error(workload, precision):
+    """Seismogram rel-L2 against the FP64 build over the full run, as
+    measured by the full-size GPU test and recorded in the committed
+    precision summary (profiles/<workload>_precision_study.md)."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_precision_study.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"seismogram_rel_l2_{precision}:"):
+            return {"value": float(line.split(":", 1)[1]), "source": os.path.relpath(path, ROOT)}
+    return None
 
 
 if __name__ == "__main__":

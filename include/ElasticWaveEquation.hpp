@@ -120,8 +120,13 @@ public:
     }
 
     // one step with an explicit source value (..._concise/ElasticWaveEquation.cu:194-206)
+    // The device injects the sample of its own step counter, so timeIndex
+    // must be the next step: steps are taken in order, each once.
     void propagate(int timeIndex, float sourceValue)
     {
+        if (timeIndex != sg_steps_done(s_))
+            throw sg::Error(SG_ERR_STATE, "propagate: timeIndex " + std::to_string(timeIndex) +
+                                              " is not the next step (" + std::to_string(sg_steps_done(s_)) + ")");
         sg::check(sg_set_source_sample(s_, timeIndex, sourceValue));
         sg::check(sg_step(s_, 1));
     }

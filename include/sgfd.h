@@ -49,7 +49,8 @@ typedef enum sg_status {
     SG_ERR_NOMEM = -3,    /* device or host allocation failed */
     SG_ERR_HIP = -4,      /* a HIP runtime call failed (see sg_last_error) */
     SG_ERR_STATE = -5,    /* call not valid in the current state */
-    SG_ERR_UNSUPPORTED = -6
+    SG_ERR_UNSUPPORTED = -6,
+    SG_ERR_UNSTABLE = -7  /* the stability check stopped stepping (sg_set_stability) */
 } sg_status;
 
 typedef enum sg_precision {
@@ -121,7 +122,10 @@ int sg_destroy(sg_solver *s);
 
 /* Host-side copy of the model (Medium buoyancy/lambda/mu of CONCISE/
  * ElasticMacro.h:379-384, copied in like run()'s H2D at .cu:145-147).
- * x-fastest over the total grid, n = nx*ny*nz values each. */
+ * x-fastest over the total grid, n = nx*ny*nz values each.  b and mu must be
+ * positive, 3 lambda + 2 mu positive.  A model whose Courant number
+ * (sg_courant) exceeds the stability limit is refused with SG_ERR_INVALID
+ * (the reference runs it to garbage). */
 int sg_set_medium(sg_solver *s, const float *buoyancy, const float *lambda, const float *mu);
 /* The model as P velocity, S velocity and density (the reference's main
  * programs form b = 1/rho, lambda and mu from a velocity and a density,
@@ -178,10 +182,37 @@ int sg_set_rounding(sg_solver *s, int mode);
 int sg_set_compensation(sg_solver *s, unsigned field_mask);
 int sg_get_field_scales(const sg_solver *s, float out[12]);
 
+/* ---- stability guard ----------------------------------------------------- */
+
+/* Largest normalised Courant number of the current model (0 before
+ * sg_set_medium): dt * max(vp) * (|c1|+...+|c5|) * sqrt(1/dx^2 + 1/dy^2 +
+ * 1/dz^2), vp = sqrt((lambda + 2 mu) b) per point.  The staggered leapfrog
+ * scheme is stable at <= 1 (10th order, dx = dy = dz: vp dt/dx <= 0.4385).
+ * No counterpart in the reference, which never checks. */
+int sg_courant(const sg_solver *s, double *courant);
+
+/* courant_limit: sg_set_medium refuses a model above it (default 1; 0 = no
+ * check).  check_interval: every that many steps of sg_step/sg_run (counted
+ * by sg_steps_done) an FP64 energy check runs and sg_step waits for it
+ * (default 0 = off); it returns SG_ERR_UNSTABLE, with the remaining steps not
+ * run, when a wavefield holds a non-finite value, or when, after the last
+ * wavelet sample above 1e-6 of the wavelet's peak, the total energy
+ * (kinetic + strain, sg_energy_full) grew by more than max_growth (> 1,
+ * default 2) since the previous check.  Batched steps are not checked.  No
+ * counterpart in the reference. */
+typedef struct sg_stability {
+    double courant_limit;
+    int check_interval;
+    double max_growth;
+} sg_stability;
+int sg_set_stability(sg_solver *s, const sg_stability *st);
+int sg_get_stability(const sg_solver *s, sg_stability *st);
+
 /* ---- stepping --------------------------------------------------------------- */
 
 /* Advances nsteps (source -> velocity -> stress -> record per step), queued on
- * the solver's stream; returns without waiting. */
+ * the solver's stream; returns without waiting (except for the stability
+ * check, when enabled: sg_set_stability). */
 int sg_step(sg_solver *s, int nsteps);
 /* time_length steps from the current state, then waits (run(), minus its
  * file output). */
@@ -204,6 +235,13 @@ int sg_get_field_f64(sg_solver *s, int field, double *out);
  * out[1] = sum of the six stress components squared.  Used to check that
  * the PML drains energy after the source stops. */
 int sg_energy(sg_solver *s, double out[2]);
+/* The same reduction with a third value: out[2] = the strain energy
+ * sum sigma:sigma/(4 mu) - lambda (tr sigma)^2 / (4 mu (3 lambda + 2 mu))
+ * (sigma:sigma counts each shear component twice), so out[0] + out[2] is the
+ * elastic energy the stability check follows.  Every sum is taken in a fixed
+ * order (per z plane, then over the planes): the result does not depend on
+ * scheduling. */
+int sg_energy_full(sg_solver *s, double out[3]);
 /* Largest |value| of each wavefield (vx vy vz sxx syy szz sxy sxz syz),
  * unscaled.  Run an FP32 solver for a pilot interval and read this to pick
  * the power-of-two storage scales of an FP16/BF16 solver
@@ -294,6 +332,12 @@ int sg_gaussian_field(int nx, int ny, int nz, double corr, unsigned long long se
  * floats. */
 int sg_read_raw_f32(const char *path, long long n, float *out);
 int sg_write_raw_f32(const char *path, long long n, const float *in);
+
+/* The normalised Courant number of a model given as n points of b, lambda,
+ * mu (sg_courant; what sg_set_medium checks), on the host in double.  The
+ * same validity checks as sg_set_medium. */
+int sg_courant_number(long long n, const float *buoyancy, const float *lambda, const float *mu, float dt, float dx,
+                      float dy, float dz, const float diff_coef[5], double *courant);
 
 /* b = 1/rho, mu = rho*vs^2, lambda = rho*vp^2 - 2*mu for n points, formed in
  * double and rounded once to float.  rho and vs must be positive and finite,

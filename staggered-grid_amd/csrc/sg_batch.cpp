@@ -26,7 +26,7 @@ hipStream_t solver_stream(const sg_solver *s);
 const Geom &solver_geom(const sg_solver *s);
 void solver_aux(const sg_solver *s, AuxDesc *a);
 void solver_add_steps(sg_solver *s, long long n);
-void solver_batch_ref(sg_solver *s, int d);
+void solver_batch_ref(sg_solver *s, hipStream_t batch_stream, int d);
 int solver_device(const sg_solver *s);
 }  // namespace sg
 
@@ -160,11 +160,19 @@ int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out)
     if (!out) return fail(SG_ERR_INVALID, "sg_batch_create: out is NULL");
     *out = nullptr;
     if (!shots || n < 1) return fail(SG_ERR_INVALID, "sg_batch_create: need at least one shot");
+    if (!shots[0]) return fail(SG_ERR_INVALID, "sg_batch_create: NULL shot");
     const int dev = solver_device(shots[0]);
     for (int i = 0; i < n; ++i) {
         if (!shots[i]) return fail(SG_ERR_INVALID, "sg_batch_create: NULL shot");
         if (solver_prec(shots[i]) != SG_FP32) return fail(SG_ERR_UNSUPPORTED, "sg_batch_create: batching is FP32 only");
         if (solver_device(shots[i]) != dev) return fail(SG_ERR_INVALID, "sg_batch_create: shots on different devices");
+        // the batched sweeps are the tiled ones, whose limits sg_create
+        // checks only for tiled solvers
+        if (solver_kernel(shots[i]) != SG_KERNEL_TILED)
+            return fail(SG_ERR_UNSUPPORTED, "sg_batch_create: shots must use the tiled sweeps (SG_KERNEL_TILED)");
+        for (int j = 0; j < i; ++j)
+            if (shots[j] == shots[i])
+                return fail(SG_ERR_INVALID, "sg_batch_create: shot " + std::to_string(i) + " is listed twice");
     }
     sg_batch *b = new (std::nothrow) sg_batch();
     if (!b) return fail(SG_ERR_NOMEM, "sg_batch_create: host allocation failed");
@@ -190,7 +198,7 @@ int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out)
                          std::string("sg_batch_create: ") + hipGetErrorString(e)));
     if (int rc = sync_descriptors(b, true)) return bail(rc);
     if (int rc = build_items(b)) return bail(rc);
-    for (auto *s : b->shots) solver_batch_ref(s, +1);
+    for (auto *s : b->shots) solver_batch_ref(s, b->stream, +1);
     *out = b;
     return SG_OK;
 }
@@ -278,7 +286,7 @@ int sg_batch_destroy(sg_batch *b)
     if (!b) return SG_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    for (auto *s : b->shots) solver_batch_ref(s, -1);
+    for (auto *s : b->shots) solver_batch_ref(s, b->stream, -1);
     free_batch(b);
     delete b;
     return SG_OK;

@@ -2,6 +2,7 @@
 // coefficients, damping profiles and the Ricker wavelet.  No device work.
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "../../include/sgfd.h"
 #include "sg_error.hpp"
@@ -220,4 +221,30 @@ extern "C" int sg_write_raw_f32(const char *path, long long n, const float *in)
     const size_t put = std::fwrite(in, sizeof(float), (size_t)n, f);
     const bool ok = std::fclose(f) == 0 && put == (size_t)n;
     return ok ? SG_OK : sg::fail(SG_ERR_INVALID, std::string("sg_write_raw_f32: write to ") + path + " failed");
+}
+
+// Largest normalised Courant number of a model: the staggered operator's
+// largest symbol is 2/h * sum |c_n| (at k h = pi, where sin((2n-1) k h / 2)
+// alternates in sign like c_n), so the leapfrog scheme is stable for
+// dt * max(vp) * sum |c_n| * sqrt(1/dx^2 + 1/dy^2 + 1/dz^2) <= 1, with
+// vp = sqrt((lambda + 2 mu) b) per point.
+extern "C" int sg_courant_number(long long n, const float *b, const float *lam, const float *mu, float dt, float dx,
+                                 float dy, float dz, const float diff_coef[5], double *courant)
+{
+    if (n < 1 || !b || !lam || !mu || !diff_coef || !courant || !(dt > 0) || !(dx > 0) || !(dy > 0) || !(dz > 0))
+        return sg::fail(SG_ERR_INVALID, "sg_courant_number: invalid arguments");
+    double vp2 = 0.0;
+    for (long long i = 0; i < n; ++i) {
+        if (!(b[i] > 0) || !std::isfinite(b[i]) || !std::isfinite(lam[i]) || !(mu[i] > 0) || !std::isfinite(mu[i]) ||
+            !(3.0 * (double)lam[i] + 2.0 * (double)mu[i] > 0))
+            return sg::fail(SG_ERR_INVALID, "buoyancy and mu must be positive and finite, lambda finite with "
+                                            "3 lambda + 2 mu > 0 (first bad index " + std::to_string(i) + ")");
+        const double v = ((double)lam[i] + 2.0 * (double)mu[i]) * (double)b[i];
+        if (v > vp2) vp2 = v;
+    }
+    double cs = 0.0;
+    for (int k = 0; k < 5; ++k) cs += std::fabs((double)diff_coef[k]);
+    const double hx = dx, hy = dy, hz = dz;
+    *courant = (double)dt * std::sqrt(vp2) * cs * std::sqrt(1.0 / (hx * hx) + 1.0 / (hy * hy) + 1.0 / (hz * hz));
+    return SG_OK;
 }

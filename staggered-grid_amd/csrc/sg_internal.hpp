@@ -202,7 +202,11 @@ hipError_t launch_inject(int prec, const void *shot, const double *amp_d, const 
 hipError_t launch_gather(int prec, const void *shot, const long long *idx, const int *comp, int nst,
                          int ncomp, int interval, long long capacity, void *out, const int *d_step,
                          hipStream_t st);
-hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st);
+// FP64 energies (kinetic, sum of stresses squared, strain energy) into
+// out3[0..2]; scratch holds ENERGY_PART doubles per updated z plane (nz - 10
+// planes); out3 may lie inside the caller's scratch beyond those
+constexpr int ENERGY_PART = 4;
+hipError_t launch_energy(int prec, const void *shot, double *scratch, double *out3, hipStream_t st);
 // local S velocity sqrt(mu*b) at every split slot (per-cell damping)
 hipError_t launch_fill_vs(int prec, const void *shot, void *vs, hipStream_t st);
 // largest |value| per wavefield (float bits, atomicMax into out9[9], which

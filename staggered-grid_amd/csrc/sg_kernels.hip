@@ -376,61 +376,99 @@ __global__ __launch_bounds__(256) void k_maxabs(const Shot<T, C> s, unsigned *ou
     if ((int)threadIdx.x < NFIELD) atomicMax(&out9[threadIdx.x], __float_as_uint(red[threadIdx.x][0]));
 }
 
+// FP64 energies over the updated region, summed in a fixed order (one block
+// per z plane of the updated region, then one block over the planes), so a
+// result does not depend on scheduling:
+//   e[0] = sum rho |v|^2 / 2   (kinetic, rho = 1/b)
+//   e[1] = sum of the six stress components squared
+//   e[2] = sum sigma:sigma / (4 mu) - lambda (tr sigma)^2 / (4 mu (3 lambda + 2 mu))
+//          (strain energy of the isotropic medium; sigma:sigma counts each
+//          shear component twice)
+// part: ENERGY_PART doubles per plane; block (64, 16).
 template <class T, class C>
-__global__ __launch_bounds__(256) void k_energy(const Shot<T, C> s, double *out)
+__global__ __launch_bounds__(1024) void k_energy_planes(const Shot<T, C> s, double *part)
 {
     const Geom &g = s.g;
-    const long long n = (long long)g.ux * g.uy * g.uz;
-    double ek = 0.0, es = 0.0;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % g.ux) + HALO;
-        const long long r = t / g.ux;
-        const int j = (int)(r % g.uy) + HALO;
-        const int k = (int)(r / g.uy) + HALO;
-        const long long o = i + j * g.sy + k * g.sz;
-        const double b = (double)Cvt<T, C>::ld(s.b[o], 1.0f / s.bscale);
-        double v2 = 0.0;
-        for (int c = 0; c < 3; ++c) {
-            const double v = (double)ldf(s, VX + c, o);
-            v2 += v * v;
-        }
-        ek += 0.5 * v2 / b;
-        for (int c = 0; c < 6; ++c) {
-            const double v = (double)ldf(s, SXX + c, o);
-            es += v * v;
+    const int k = (int)blockIdx.x + HALO;
+    const float ib = 1.0f / s.bscale, il = 1.0f / s.lscale, im = 1.0f / s.mscale;
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+    for (int j = HALO + (int)threadIdx.y; j < g.ny - HALO; j += (int)blockDim.y) {
+        for (int i = HALO + (int)threadIdx.x; i < g.nx - HALO; i += (int)blockDim.x) {
+            const long long o = i + j * g.sy + k * g.sz;
+            const double b = (double)Cvt<T, C>::ld(s.b[o], ib);
+            const double lam = (double)Cvt<T, C>::ld(s.lam[o], il);
+            const double mu = (double)Cvt<T, C>::ld(s.mu[o], im);
+            double v2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double v = (double)ldf(s, VX + c, o);
+                v2 += v * v;
+            }
+            double n2 = 0.0, sh2 = 0.0, tr = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double v = (double)ldf(s, SXX + c, o);
+                n2 += v * v;
+                tr += v;
+            }
+            for (int c = 3; c < 6; ++c) {
+                const double v = (double)ldf(s, SXX + c, o);
+                sh2 += v * v;
+            }
+            e0 += 0.5 * v2 / b;
+            e1 += n2 + sh2;
+            e2 += (n2 + 2.0 * sh2) / (4.0 * mu) - lam * tr * tr / (4.0 * mu * (3.0 * lam + 2.0 * mu));
         }
     }
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = ek;
-    red[1][threadIdx.x] = es;
+    __shared__ double red[3][1024];
+    const int t = (int)(threadIdx.x + threadIdx.y * blockDim.x);
+    red[0][t] = e0;
+    red[1][t] = e1;
+    red[2][t] = e2;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + w];
-            red[1][threadIdx.x] += red[1][threadIdx.x + w];
-        }
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 3; ++q) red[q][t] += red[q][t + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        atomicAdd(&out[0], red[0][0]);
-        atomicAdd(&out[1], red[1][0]);
-    }
+    if (t < 3) part[(size_t)blockIdx.x * ENERGY_PART + t] = red[t][0];
 }
 
-// streaming copy, four 16-byte loads in flight per thread
-__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n4)
+// out[0..2] = the plane partials summed in plane order by a fixed tree
+__global__ __launch_bounds__(256) void k_energy_sum(const double *part, int nplanes, double *out)
 {
-    const long long st = (long long)gridDim.x * blockDim.x;
-    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    for (; t + 3 * st < n4; t += 4 * st) {
-        const float4 a = src[t], b = src[t + st], c = src[t + 2 * st], d = src[t + 3 * st];
-        dst[t] = a;
-        dst[t + st] = b;
-        dst[t + 2 * st] = c;
-        dst[t + 3 * st] = d;
+    __shared__ double red[3][256];
+    const int t = (int)threadIdx.x;
+    double e[3] = {0.0, 0.0, 0.0};
+    for (int p = t; p < nplanes; p += 256)
+        for (int q = 0; q < 3; ++q) e[q] += part[(size_t)p * ENERGY_PART + q];
+    for (int q = 0; q < 3; ++q) red[q][t] = e[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 3; ++q) red[q][t] += red[q][t + w];
+        __syncthreads();
     }
-    for (; t < n4; t += st) dst[t] = src[t];
+    if (t < 3) out[t] = red[t][0];
+}
+
+// streaming copy: one pass, each thread loads four 16-byte values of its
+// block's contiguous 16 KiB before storing them nontemporally (measured on
+// one MI355X, 1 GiB each way: 5.7 TB/s, against 4.2-4.5 TB/s for a
+// grid-stride loop with ordinary stores)
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(const v4f *__restrict__ src, v4f *__restrict__ dst, long long n4)
+{
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    v4f v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const long long t = base + q * 256;
+        if (t < n4) v[q] = src[t];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const long long t = base + q * 256;
+        if (t < n4) __builtin_nontemporal_store(v[q], &dst[t]);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_zero(uint32_t *p, long long n)
@@ -553,16 +591,18 @@ hipError_t launch_gather(int prec, const void *shot, const long long *idx, const
 }
 
 template <class T, class C>
-static hipError_t energy_t(const void *shot, double *out2, hipStream_t st)
+static hipError_t energy_t(const void *shot, double *scratch, double *out3, hipStream_t st)
 {
     const Shot<T, C> &s = *static_cast<const Shot<T, C> *>(shot);
-    hipLaunchKernelGGL((k_energy<T, C>), dim3(1024), dim3(256), 0, st, s, out2);
+    const int np = s.g.nz - 2 * HALO;
+    hipLaunchKernelGGL((k_energy_planes<T, C>), dim3(np), dim3(64, 16), 0, st, s, scratch);
+    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(256), 0, st, (const double *)scratch, np, out3);
     return hipGetLastError();
 }
 
-hipError_t launch_energy(int prec, const void *shot, double *out2, hipStream_t st)
+hipError_t launch_energy(int prec, const void *shot, double *scratch, double *out3, hipStream_t st)
 {
-    SG_DISPATCH(prec, energy_t, shot, out2, st)
+    SG_DISPATCH(prec, energy_t, shot, scratch, out3, st)
 }
 
 template <class T, class C>
@@ -608,9 +648,11 @@ hipError_t launch_gather_batch(const void *shots_dev, const AuxDesc *aux, int ns
 
 hipError_t launch_copy(const float *src, float *dst, long long n, hipStream_t st)
 {
-    const long long n4 = n / 4;
-    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, reinterpret_cast<const float4 *>(src),
-                       reinterpret_cast<float4 *>(dst), n4);
+    const long long n4 = n / 4, blocks = (n4 + 1023) / 1024;
+    if (blocks <= 0) return hipSuccess;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, reinterpret_cast<const v4f *>(src),
+                       reinterpret_cast<v4f *>(dst), n4);
     return hipGetLastError();
 }
 

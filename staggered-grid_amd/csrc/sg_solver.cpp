@@ -118,12 +118,24 @@ struct sg_solver {
     int nst = 0, ncomp = 0, interval = 1;
     long long st_cap = 0;
 
+    // energy reductions: ENERGY_PART partials per updated z plane, then the
+    // results (4 doubles), then the nine max-abs counters (unsigned)
     double *d_energy = nullptr;
+    // stability guard (sg_set_stability): Courant limit checked by
+    // sg_set_medium, the FP64 energy check every check_interval steps
+    double courant = 0.0;        // of the current model (0: none set yet)
+    double courant_limit = 1.0;  // 0: no check
+    int check_interval = 0;
+    double max_growth = 2.0;
+    double last_energy = -1.0;   // total energy at the previous check after the source ended
+    int src_last = -1;           // last wavelet sample above 1e-6 of its peak
     int round_mode = SG_ROUND_NEAREST;
     void *comp[NFIELD] = {};  // compensation arrays (16-bit storage, see sg_set_compensation)
     unsigned comp_mask = 0;
     long long steps_done = 0;
-    int batch_refs = 0;  // live batches holding this solver's device pointers
+    // streams of the live batches that step this solver (sg_batch_create);
+    // setters, getters and single steps wait for them as for `stream`
+    std::vector<hipStream_t> batch_streams;
     std::vector<double> damp[3];
     bool damp_local = false;  // damp[] holds g of d = g * sqrt(mu*b) (sg_set_damping_local)
     void *vs = nullptr;       // local S velocity per split slot (compute type)
@@ -395,6 +407,46 @@ void free_all(sg_solver *s)
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
+// Device buffers allocated together by a setter: all freed on scope exit
+// unless keep() hands them over, so a failed call leaves the solver as it was.
+struct DevBuffers {
+    std::vector<void *> p;
+    int alloc(void **out, size_t bytes, const char *who)
+    {
+        *out = nullptr;
+        const hipError_t e = hipMalloc(out, bytes);
+        if (e != hipSuccess) {
+            *out = nullptr;
+            (void)hipGetLastError();
+            return fail(e == hipErrorOutOfMemory ? SG_ERR_NOMEM : SG_ERR_HIP,
+                        std::string(who) + ": device allocation of " + std::to_string(bytes) +
+                            " bytes failed: " + hipGetErrorString(e));
+        }
+        p.push_back(*out);
+        return SG_OK;
+    }
+    void keep() { p.clear(); }
+    ~DevBuffers()
+    {
+        for (void *q : p) (void)hipFree(q);
+    }
+};
+
+// Waits for the solver's stream and for every batch stream that steps this
+// solver: a setter or getter must not overlap batched sweeps that read or
+// write the same buffers.
+int quiesce(sg_solver *s)
+{
+    SG_HIP(hipStreamSynchronize(s->stream));
+    for (hipStream_t b : s->batch_streams) SG_HIP(hipStreamSynchronize(b));
+    return SG_OK;
+}
+
+#define SG_QUIESCE(s)                     \
+    do {                                  \
+        if (int rc_q_ = quiesce(s)) return rc_q_; \
+    } while (0)
+
 int check_solver(const sg_solver *s)
 {
     if (!s) return fail(SG_ERR_INVALID, "solver handle is NULL");
@@ -500,7 +552,7 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     SG_TRY(hipMemsetAsync(s->saved, 0, 8 * sT, s->stream));
     SG_TRY(hipMalloc(&s->d_step, sizeof(int)));
     SG_TRY(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
-    SG_TRY(hipMalloc(&s->d_energy, 2 * sizeof(double) + NFIELD * sizeof(unsigned)));
+    SG_TRY(hipMalloc(&s->d_energy, (size_t)(ENERGY_PART * g.uz + 4) * sizeof(double) + NFIELD * sizeof(unsigned)));
     SG_TRY(hipMalloc(&s->d_shot, sizeof(Shot<double, double>)));
     SG_TRY(hipStreamSynchronize(s->stream));
 #undef SG_TRY
@@ -521,7 +573,7 @@ int sg_create(const sg_config *cfg, sg_solver **out)
 int sg_destroy(sg_solver *s)
 {
     if (!s) return SG_OK;
-    if (s->batch_refs > 0)
+    if (!s->batch_streams.empty())
         return fail(SG_ERR_STATE, "sg_destroy: the solver belongs to a live batch; destroy the batch first");
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -539,12 +591,19 @@ int sg_set_medium(sg_solver *s, const float *b, const float *lam, const float *m
     const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz;
     const float *src[3] = {b, lam, mu};
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
-    for (size_t i = 0; i < N; ++i) {
-        if (!(b[i] > 0) || !std::isfinite(b[i]) || !std::isfinite(lam[i]) || !(mu[i] > 0) || !std::isfinite(mu[i]))
-            return fail(SG_ERR_INVALID, "sg_set_medium: buoyancy and mu must be positive and finite, lambda finite "
-                                        "(first bad index " + std::to_string(i) + ")");
+    SG_QUIESCE(s);
+    double courant = 0.0;
+    if (int rc = sg_courant_number((long long)N, b, lam, mu, s->cfg.dt, s->cfg.dx, s->cfg.dy, s->cfg.dz,
+                                   s->cfg.diff_coef, &courant))
+        return fail(rc, std::string("sg_set_medium: ") + sg_last_error());
+    if (s->courant_limit > 0 && courant > s->courant_limit) {
+        char m[256];
+        snprintf(m, sizeof m, "sg_set_medium: unstable model: Courant number %.6g exceeds the limit %.6g "
+                 "(dt * max vp * sum|c_n| * sqrt(sum 1/h^2); reduce dt or raise the limit with sg_set_stability)",
+                 courant, s->courant_limit);
+        return fail(SG_ERR_INVALID, m);
     }
+    s->courant = courant;
     if (s->prec == SG_FP32) {
         for (int m = 0; m < 3; ++m) SG_HIP(hipMemcpy(s->model[m], src[m], N * 4, hipMemcpyHostToDevice));
     } else if (s->prec == SG_FP64) {
@@ -598,7 +657,7 @@ int sg_set_field_scales(sg_solver *s, const float scales[9])
     }
     if (s->steps_done != 0) return fail(SG_ERR_STATE, "sg_set_field_scales: set scales before stepping (or after sg_reset)");
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     for (int f = 0; f < NFIELD; ++f) s->fscale[f] = s->prec == SG_FP16 || s->prec == SG_BF16 ? scales[f] : 1.0f;
     return refresh_shot(s);
 }
@@ -624,7 +683,7 @@ int sg_set_damping(sg_solver *s, const double *dx, const double *dy, const doubl
             if (!std::isfinite(d3[a][i]) || d3[a][i] < 0) return fail(SG_ERR_INVALID, "sg_set_damping: d must be finite and >= 0");
     }
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     for (int a = 0; a < 3; ++a) s->damp[a].assign(d3[a], d3[a] + n3[a]);
     const bool was_local = s->damp_local;
     s->damp_local = false;
@@ -639,7 +698,7 @@ static int fill_vs(sg_solver *s)
     if (!s->vs) SG_HIP(hipMalloc(&s->vs, (size_t)s->g.nslots * compute_size(s->prec)));
     if (int rc = refresh_shot(s)) return rc;
     SG_HIP(launch_fill_vs(s->prec, s->shot, s->vs, s->stream));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     return SG_OK;
 }
 
@@ -655,7 +714,7 @@ int sg_set_damping_local(sg_solver *s, const double *gx, const double *gy, const
                 return fail(SG_ERR_INVALID, "sg_set_damping_local: g must be finite and >= 0");
     }
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     for (int a = 0; a < 3; ++a) s->damp[a].assign(g3[a], g3[a] + n3[a]);
     s->damp_local = true;
     if (int rc = upload_tables(s)) return rc;
@@ -670,32 +729,42 @@ int sg_set_source(sg_solver *s, int x, int y, int z, const float *wavelet, int n
         return fail(SG_ERR_RANGE, "sg_set_source: source must lie in the updated region [5, n-5)");
     if (n < 0 || (n > 0 && !wavelet)) return fail(SG_ERR_INVALID, "sg_set_source: bad wavelet");
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
-    if (s->amp_d) SG_HIP(hipFree(s->amp_d));
-    if (s->amp_acc) SG_HIP(hipFree(s->amp_acc));
-    s->amp_d = nullptr;
-    s->amp_acc = nullptr;
-    s->namp = n;
+    SG_QUIESCE(s);
+    // new buffers first; the solver changes only once they are complete
+    DevBuffers nb;
+    double *amp_d = nullptr;
+    void *amp_acc = nullptr;
     if (n > 0) {
         // TRANSIENT: pow(dt, 2) * s(it) in double (ElasticWaveEquation3D.cpp:123)
         std::vector<double> ad(n);
         const double dt = (double)s->cfg.dt;
         for (int i = 0; i < n; ++i) ad[i] = dt * dt * (double)wavelet[i];
-        SG_HIP(hipMalloc(&s->amp_d, n * sizeof(double)));
-        SG_HIP(hipMemcpy(s->amp_d, ad.data(), n * sizeof(double), hipMemcpyHostToDevice));
-        // ACCUMULATE: deltaT * deltaT * sourceValue in the compute type (..._concise/.cu:216)
         const size_t sC = compute_size(s->prec);
-        SG_HIP(hipMalloc(&s->amp_acc, n * sC));
+        if (int rc = nb.alloc((void **)&amp_d, n * sizeof(double), "sg_set_source")) return rc;
+        if (int rc = nb.alloc(&amp_acc, n * sC, "sg_set_source")) return rc;
+        SG_HIP(hipMemcpy(amp_d, ad.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        // ACCUMULATE: deltaT * deltaT * sourceValue in the compute type (..._concise/.cu:216)
         if (sC == 8) {
-            std::vector<double> a(n);
-            for (int i = 0; i < n; ++i) a[i] = dt * dt * (double)wavelet[i];
-            SG_HIP(hipMemcpy(s->amp_acc, a.data(), n * 8, hipMemcpyHostToDevice));
+            SG_HIP(hipMemcpy(amp_acc, ad.data(), n * 8, hipMemcpyHostToDevice));
         } else {
             std::vector<float> a(n);
             const float dtf = s->cfg.dt;
             for (int i = 0; i < n; ++i) a[i] = dtf * dtf * wavelet[i];
-            SG_HIP(hipMemcpy(s->amp_acc, a.data(), n * 4, hipMemcpyHostToDevice));
+            SG_HIP(hipMemcpy(amp_acc, a.data(), n * 4, hipMemcpyHostToDevice));
         }
+    }
+    nb.keep();
+    if (s->amp_d) (void)hipFree(s->amp_d);
+    if (s->amp_acc) (void)hipFree(s->amp_acc);
+    s->amp_d = amp_d;
+    s->amp_acc = amp_acc;
+    s->namp = n;
+    {
+        float peak = 0.0f;
+        for (int i = 0; i < n; ++i) peak = std::max(peak, std::fabs(wavelet[i]));
+        s->src_last = -1;
+        for (int i = 0; i < n; ++i)
+            if (std::fabs(wavelet[i]) > 1e-6f * peak) s->src_last = i;
     }
     s->src = x + (long long)y * g.sy + (long long)z * g.sz;
     return refresh_shot(s);
@@ -708,15 +777,15 @@ int sg_set_source_sample(sg_solver *s, int it, float value)
     SG_HIP(hipSetDevice(s->device));
     const double dt = (double)s->cfg.dt;
     const double ad = dt * dt * (double)value;
-    // ordered after the steps already queued, before the ones queued later
-    SG_HIP(hipMemcpyAsync(s->amp_d + it, &ad, sizeof ad, hipMemcpyHostToDevice, s->stream));
+    // after every step already queued on this solver or a batch holding it
+    SG_QUIESCE(s);
+    SG_HIP(hipMemcpy(s->amp_d + it, &ad, sizeof ad, hipMemcpyHostToDevice));
     if (compute_size(s->prec) == 8) {
-        SG_HIP(hipMemcpyAsync(static_cast<double *>(s->amp_acc) + it, &ad, 8, hipMemcpyHostToDevice, s->stream));
+        SG_HIP(hipMemcpy(static_cast<double *>(s->amp_acc) + it, &ad, 8, hipMemcpyHostToDevice));
     } else {
         const float af = s->cfg.dt * s->cfg.dt * value;
-        SG_HIP(hipMemcpyAsync(static_cast<float *>(s->amp_acc) + it, &af, 4, hipMemcpyHostToDevice, s->stream));
+        SG_HIP(hipMemcpy(static_cast<float *>(s->amp_acc) + it, &af, 4, hipMemcpyHostToDevice));
     }
-    SG_HIP(hipStreamSynchronize(s->stream));
     return SG_OK;
 }
 
@@ -738,29 +807,79 @@ int sg_set_stations(sg_solver *s, const int *xyz, int n, unsigned comp_mask, int
     for (int f = 0; f < NFIELD; ++f)
         if (comp_mask & (1u << f)) comp.push_back(f);
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
-    if (s->st_idx) SG_HIP(hipFree(s->st_idx));
-    if (s->st_comp) SG_HIP(hipFree(s->st_comp));
-    if (s->st_out) SG_HIP(hipFree(s->st_out));
-    s->st_idx = nullptr;
-    s->st_comp = nullptr;
-    s->st_out = nullptr;
+    SG_QUIESCE(s);
+    // new buffers first; the solver changes only once they are complete
+    DevBuffers nb;
+    long long *st_idx = nullptr;
+    int *st_comp = nullptr;
+    void *st_out = nullptr;
+    long long cap = 0;
+    if (n > 0) {
+        const long long steps = std::max(1, s->cfg.time_length);
+        cap = (steps + interval - 1) / interval;
+        const size_t bytes = (size_t)cap * n * comp.size() * compute_size(s->prec);
+        if (int rc = nb.alloc((void **)&st_idx, n * sizeof(long long), "sg_set_stations")) return rc;
+        if (int rc = nb.alloc((void **)&st_comp, comp.size() * sizeof(int), "sg_set_stations")) return rc;
+        if (int rc = nb.alloc(&st_out, bytes, "sg_set_stations")) return rc;
+        SG_HIP(hipMemcpy(st_idx, idx.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+        SG_HIP(hipMemcpy(st_comp, comp.data(), comp.size() * sizeof(int), hipMemcpyHostToDevice));
+        SG_HIP(launch_fill_zero(st_out, (long long)bytes, s->stream));
+        SG_HIP(hipStreamSynchronize(s->stream));
+    }
+    nb.keep();
+    if (s->st_idx) (void)hipFree(s->st_idx);
+    if (s->st_comp) (void)hipFree(s->st_comp);
+    if (s->st_out) (void)hipFree(s->st_out);
+    s->st_idx = st_idx;
+    s->st_comp = st_comp;
+    s->st_out = st_out;
     s->nst = n;
     s->ncomp = (int)comp.size();
     s->interval = interval;
-    s->st_cap = 0;
-    if (n > 0) {
-        const long long steps = std::max(1, s->cfg.time_length);
-        s->st_cap = (steps + interval - 1) / interval;
-        SG_HIP(hipMalloc(&s->st_idx, n * sizeof(long long)));
-        SG_HIP(hipMemcpy(s->st_idx, idx.data(), n * sizeof(long long), hipMemcpyHostToDevice));
-        SG_HIP(hipMalloc(&s->st_comp, comp.size() * sizeof(int)));
-        SG_HIP(hipMemcpy(s->st_comp, comp.data(), comp.size() * sizeof(int), hipMemcpyHostToDevice));
-        const size_t bytes = (size_t)s->st_cap * n * comp.size() * compute_size(s->prec);
-        SG_HIP(hipMalloc(&s->st_out, bytes));
-        SG_HIP(hipMemset(s->st_out, 0, bytes));
-    }
+    s->st_cap = cap;
     s->graph_valid = false;
+    return SG_OK;
+}
+
+// The stability check after the step just completed: FP64 energies of the
+// current state; SG_ERR_UNSTABLE for a non-finite wavefield, or for energy
+// growth beyond max_growth between two checks once the source has stopped.
+static int stability_check(sg_solver *s)
+{
+    double *res = s->d_energy + (size_t)ENERGY_PART * s->g.uz;
+    double e[3];
+    SG_HIP(launch_energy(s->prec, s->shot, s->d_energy, res, s->stream));
+    SG_HIP(hipMemcpyAsync(e, res, sizeof e, hipMemcpyDeviceToHost, s->stream));
+    SG_HIP(hipStreamSynchronize(s->stream));
+    char m[320];
+    if (!std::isfinite(e[0]) || !std::isfinite(e[1]) || !std::isfinite(e[2])) {
+        snprintf(m, sizeof m, "sg_step: unstable: a wavefield holds a non-finite value after step %lld",
+                 s->steps_done);
+        return fail(SG_ERR_UNSTABLE, m);
+    }
+    const double tot = e[0] + e[2];
+    if (s->steps_done > (long long)s->src_last + 1) {
+        if (s->last_energy > 0 && tot > s->max_growth * s->last_energy) {
+            snprintf(m, sizeof m, "sg_step: unstable: energy grew from %.6g to %.6g (x%.3g, limit x%.3g) in the "
+                     "%d steps to step %lld, after the source stopped", s->last_energy, tot, tot / s->last_energy,
+                     s->max_growth, s->check_interval, s->steps_done);
+            return fail(SG_ERR_UNSTABLE, m);
+        }
+        s->last_energy = tot;
+    }
+    return SG_OK;
+}
+
+static int enqueue_steps(sg_solver *s, int m)
+{
+    if (s->use_graph) {
+        if (!s->graph_valid)
+            if (int rc = build_graph(s)) return rc;
+        for (int i = 0; i < m; ++i) SG_HIP(hipGraphLaunch(s->gexec, s->stream));
+    } else {
+        for (int i = 0; i < m; ++i)
+            if (int rc = enqueue_step(s, nullptr, nullptr, nullptr)) return rc;
+    }
     return SG_OK;
 }
 
@@ -769,16 +888,19 @@ int sg_step(sg_solver *s, int nsteps)
     if (int rc = check_solver(s)) return rc;
     if (nsteps < 0) return fail(SG_ERR_INVALID, "sg_step: nsteps < 0");
     SG_HIP(hipSetDevice(s->device));
+    // a shot stepped on its own after batched steps: those finish first
+    for (hipStream_t b : s->batch_streams) SG_HIP(hipStreamSynchronize(b));
     if (int rc = ensure_worklists(s)) return rc;
-    if (s->use_graph) {
-        if (!s->graph_valid)
-            if (int rc = build_graph(s)) return rc;
-        for (int i = 0; i < nsteps; ++i) SG_HIP(hipGraphLaunch(s->gexec, s->stream));
-    } else {
-        for (int i = 0; i < nsteps; ++i)
-            if (int rc = enqueue_step(s, nullptr, nullptr, nullptr)) return rc;
+    for (int done = 0; done < nsteps;) {
+        int m = nsteps - done;
+        const int ci = s->check_interval;
+        if (ci > 0) m = (int)std::min<long long>(m, ci - s->steps_done % ci);
+        if (int rc = enqueue_steps(s, m)) return rc;
+        s->steps_done += m;
+        done += m;
+        if (ci > 0 && s->steps_done % ci == 0)
+            if (int rc = stability_check(s)) return rc;
     }
-    s->steps_done += nsteps;
     return SG_OK;
 }
 
@@ -793,7 +915,7 @@ int sg_synchronize(sg_solver *s)
 {
     if (int rc = check_solver(s)) return rc;
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     return SG_OK;
 }
 
@@ -801,7 +923,7 @@ int sg_reset(sg_solver *s)
 {
     if (int rc = check_solver(s)) return rc;
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz, sT = storage_size(s->prec);
     for (int f = 0; f < NFIELD; ++f) {
         SG_HIP(hipMemsetAsync(s->fields[f], 0, N * sT, s->stream));
@@ -811,8 +933,9 @@ int sg_reset(sg_solver *s)
     SG_HIP(hipMemsetAsync(s->d_step, 0, sizeof(int), s->stream));
     if (s->st_out)
         SG_HIP(hipMemsetAsync(s->st_out, 0, (size_t)s->st_cap * s->nst * s->ncomp * compute_size(s->prec), s->stream));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     s->steps_done = 0;
+    s->last_energy = -1.0;
     return SG_OK;
 }
 
@@ -827,7 +950,7 @@ int sg_get_stations(sg_solver *s, float *out, long long count, long long *n_out)
     if (!out || count <= 0 || avail == 0) return SG_OK;
     const long long n = std::min(count, avail);
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     if (compute_size(s->prec) == 4) {
         SG_HIP(hipMemcpy(out, s->st_out, n * 4, hipMemcpyDeviceToHost));
     } else {
@@ -844,7 +967,7 @@ static int get_field_any(sg_solver *s, int field, float *outf, double *outd)
     if (field < 0 || field >= NFIELD) return fail(SG_ERR_RANGE, "sg_get_field: bad field id");
     const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz;
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     if (s->prec == SG_FP32) {
         if (outf) {
             SG_HIP(hipMemcpy(outf, s->fields[field], N * 4, hipMemcpyDeviceToHost));
@@ -897,10 +1020,54 @@ int sg_energy(sg_solver *s, double out[2])
     if (int rc = check_solver(s)) return rc;
     if (!out) return fail(SG_ERR_INVALID, "sg_energy: out is NULL");
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipMemsetAsync(s->d_energy, 0, 2 * sizeof(double), s->stream));
-    SG_HIP(launch_energy(s->prec, s->shot, s->d_energy, s->stream));
-    SG_HIP(hipMemcpyAsync(out, s->d_energy, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
+    double *res = s->d_energy + (size_t)ENERGY_PART * s->g.uz;
+    SG_HIP(launch_energy(s->prec, s->shot, s->d_energy, res, s->stream));
+    SG_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    SG_QUIESCE(s);
+    return SG_OK;
+}
+
+int sg_energy_full(sg_solver *s, double out[3])
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!out) return fail(SG_ERR_INVALID, "sg_energy_full: out is NULL");
+    SG_HIP(hipSetDevice(s->device));
+    SG_QUIESCE(s);
+    double *res = s->d_energy + (size_t)ENERGY_PART * s->g.uz;
+    SG_HIP(launch_energy(s->prec, s->shot, s->d_energy, res, s->stream));
+    SG_HIP(hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    SG_QUIESCE(s);
+    return SG_OK;
+}
+
+int sg_set_stability(sg_solver *s, const sg_stability *st)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!st) return fail(SG_ERR_INVALID, "sg_set_stability: NULL");
+    if (!(st->courant_limit >= 0) || !std::isfinite(st->courant_limit) || st->check_interval < 0 ||
+        !(st->max_growth > 1) || !std::isfinite(st->max_growth))
+        return fail(SG_ERR_INVALID, "sg_set_stability: need courant_limit >= 0, check_interval >= 0, max_growth > 1");
+    s->courant_limit = st->courant_limit;
+    s->check_interval = st->check_interval;
+    s->max_growth = st->max_growth;
+    s->last_energy = -1.0;
+    return SG_OK;
+}
+
+int sg_get_stability(const sg_solver *s, sg_stability *st)
+{
+    if (!s || !st) return fail(SG_ERR_INVALID, "sg_get_stability: NULL");
+    st->courant_limit = s->courant_limit;
+    st->check_interval = s->check_interval;
+    st->max_growth = s->max_growth;
+    return SG_OK;
+}
+
+int sg_courant(const sg_solver *s, double *courant)
+{
+    if (!s || !courant) return fail(SG_ERR_INVALID, "sg_courant: NULL");
+    *courant = s->courant;
     return SG_OK;
 }
 
@@ -914,7 +1081,7 @@ int sg_set_compensation(sg_solver *s, unsigned field_mask)
         return fail(SG_ERR_UNSUPPORTED, "sg_set_compensation: needs the tiled sweeps");
     if (s->steps_done != 0) return fail(SG_ERR_STATE, "sg_set_compensation: set it before stepping (or after sg_reset)");
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     const size_t N = (size_t)s->g.nx * s->g.ny * s->g.nz, sT = storage_size(s->prec);
     for (int f = 0; f < NFIELD; ++f) {
         const bool on = (field_mask >> f) & 1u;
@@ -931,7 +1098,7 @@ int sg_set_compensation(sg_solver *s, unsigned field_mask)
         }
         if (s->comp[f]) SG_HIP(hipMemsetAsync(s->comp[f], 0, N * sT, s->stream));
     }
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     s->comp_mask = field_mask;
     s->graph_valid = false;
     return refresh_shot(s);
@@ -942,7 +1109,7 @@ int sg_set_streams(sg_solver *s, int nstreams)
     if (int rc = check_solver(s)) return rc;
     if (nstreams < 1 || nstreams > 4) return fail(SG_ERR_INVALID, "sg_set_streams: nstreams must be 1 to 4");
     SG_HIP(hipSetDevice(s->device));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     s->nstreams = nstreams;
     s->graph_valid = false;
     return SG_OK;
@@ -953,6 +1120,8 @@ int sg_set_rounding(sg_solver *s, int mode)
     if (int rc = check_solver(s)) return rc;
     if (mode != SG_ROUND_NEAREST && mode != SG_ROUND_STOCHASTIC)
         return fail(SG_ERR_INVALID, "sg_set_rounding: mode must be SG_ROUND_NEAREST or SG_ROUND_STOCHASTIC");
+    SG_HIP(hipSetDevice(s->device));
+    SG_QUIESCE(s);
     s->round_mode = mode;
     return refresh_shot(s);
 }
@@ -962,12 +1131,13 @@ int sg_field_maxabs(sg_solver *s, float out[9])
     if (int rc = check_solver(s)) return rc;
     if (!out) return fail(SG_ERR_INVALID, "sg_field_maxabs: out is NULL");
     SG_HIP(hipSetDevice(s->device));
-    unsigned *d = reinterpret_cast<unsigned *>(s->d_energy + 2);
+    SG_QUIESCE(s);
+    unsigned *d = reinterpret_cast<unsigned *>(s->d_energy + (size_t)ENERGY_PART * s->g.uz + 4);
     unsigned h[NFIELD];
     SG_HIP(launch_fill_zero(d, NFIELD * sizeof(unsigned), s->stream));
     SG_HIP(launch_maxabs(s->prec, s->shot, d, s->stream));
     SG_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s->stream));
-    SG_HIP(hipStreamSynchronize(s->stream));
+    SG_QUIESCE(s);
     for (int f = 0; f < NFIELD; ++f) std::memcpy(&out[f], &h[f], sizeof(float));
     return SG_OK;
 }
@@ -1162,7 +1332,15 @@ int solver_device(const sg_solver *s) { return s->device; }
 hipStream_t solver_stream(const sg_solver *s) { return s->stream; }
 const Geom &solver_geom(const sg_solver *s) { return s->g; }
 void solver_add_steps(sg_solver *s, long long n) { s->steps_done += n; }
-void solver_batch_ref(sg_solver *s, int d) { s->batch_refs += d; }
+void solver_batch_ref(sg_solver *s, hipStream_t batch_stream, int d)
+{
+    auto &v = s->batch_streams;
+    if (d > 0) v.push_back(batch_stream);
+    else {
+        auto it = std::find(v.begin(), v.end(), batch_stream);
+        if (it != v.end()) v.erase(it);
+    }
+}
 void solver_aux(const sg_solver *s, AuxDesc *a)
 {
     a->amp_d = s->amp_d;

@@ -65,7 +65,12 @@ class _Spectrum(ctypes.Structure):
                 ("l", ctypes.c_double)]
 
 
+class _Stability(ctypes.Structure):
+    _fields_ = [("courant_limit", ctypes.c_double), ("check_interval", ctypes.c_int), ("max_growth", ctypes.c_double)]
+
+
 SPECTRA = {"reference": 0, "gaussian": 1, "von_karman": 2}
+ERR_UNSTABLE = -7
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
 _LLP = ctypes.POINTER(ctypes.c_longlong)
@@ -94,6 +99,12 @@ API = {
     "sg_get_field": (ctypes.c_int, [_VP, ctypes.c_int, _FP]),
     "sg_get_field_f64": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     "sg_energy": (ctypes.c_int, [_VP, _DP]),
+    "sg_energy_full": (ctypes.c_int, [_VP, _DP]),
+    "sg_courant": (ctypes.c_int, [_VP, _DP]),
+    "sg_courant_number": (ctypes.c_int, [ctypes.c_longlong, _FP, _FP, _FP, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_float, _FP, _DP]),
+    "sg_set_stability": (ctypes.c_int, [_VP, ctypes.POINTER(_Stability)]),
+    "sg_get_stability": (ctypes.c_int, [_VP, ctypes.POINTER(_Stability)]),
     "sg_field_maxabs": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "sg_set_rounding": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_set_damping_local": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
@@ -189,6 +200,18 @@ def elastic_from_velocity(vp, vs, rho):
     out = [np.empty_like(arrs[0]) for _ in range(3)]
     _check(lib().sg_elastic_from_velocity(arrs[0].size, *[_fptr(a) for a in arrs], *[_fptr(o) for o in out]))
     return tuple(out)
+
+
+def courant_number(b, lam, mu, dt, dx, dy, dz, coef) -> float:
+    """Normalised Courant number of a model (stable at <= 1): dt * max(vp) *
+    sum|c_n| * sqrt(1/dx^2 + 1/dy^2 + 1/dz^2), computed by the library on the
+    host."""
+    arrs = [np.ascontiguousarray(a, np.float32).ravel() for a in (b, lam, mu)]
+    c = np.ascontiguousarray(coef, np.float32)
+    out = ctypes.c_double()
+    _check(lib().sg_courant_number(arrs[0].size, *[_fptr(a) for a in arrs], dt, dx, dy, dz, _fptr(c),
+                                   ctypes.byref(out)))
+    return out.value
 
 
 def read_vdat(path, nx, ny, nz) -> np.ndarray:
@@ -409,6 +432,31 @@ class Solver:
         out = np.zeros(2, np.float64)
         _check(lib().sg_energy(self._h, out.ctypes.data_as(_DP)))
         return out
+
+    def energy_full(self):
+        """(kinetic, sum of stresses squared, strain energy), FP64, summed in
+        a fixed order."""
+        out = np.zeros(3, np.float64)
+        _check(lib().sg_energy_full(self._h, out.ctypes.data_as(_DP)))
+        return out
+
+    def courant(self) -> float:
+        """Normalised Courant number of the current model (stable at <= 1)."""
+        c = ctypes.c_double()
+        _check(lib().sg_courant(self._h, ctypes.byref(c)))
+        return c.value
+
+    def set_stability(self, courant_limit=1.0, check_interval=0, max_growth=2.0):
+        """Courant limit enforced by set_medium (0: none) and the FP64 energy
+        check every check_interval steps (0: off); a failed check raises
+        SgError with code ERR_UNSTABLE."""
+        st = _Stability(float(courant_limit), int(check_interval), float(max_growth))
+        _check(lib().sg_set_stability(self._h, ctypes.byref(st)))
+
+    def stability(self):
+        st = _Stability()
+        _check(lib().sg_get_stability(self._h, ctypes.byref(st)))
+        return st.courant_limit, st.check_interval, st.max_growth
 
     def set_streams(self, n):
         """Streams the work-lists of a sweep are spread over (1, 2 or 3)."""

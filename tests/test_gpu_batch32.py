@@ -1,0 +1,92 @@
+"""The batched workload at its full size: 32 shots, nx, ny ~ U{128..320},
+nz ~ U{96..256}, a 12-cell PML, each with its own seeded random medium
+(mean Vp U[2500, 3500], 5 %, correlation 6 cells) and Ricker source
+(f0 U[10, 20] Hz), 16 x 16 surface receivers recording vx, vy, vz.  Each
+shot replaces one run() of the reference CUDA driver
+(3DGPUwithPML_1DArraySingleGPU_concise/ElasticWaveEquation.cu:140-185).
+
+* 100 batched steps (the shots' work-lists in shared launches, one step
+  replayed from a hipGraph): every shot's nine fields and traces are
+  bit-identical to the same shot stepped alone.
+* The two largest shots over the full 800 steps: FP32 against the FP64 build
+  of the same kernels, rel-L2 <= 1e-5 on the receivers; the final fields,
+  which hold the FP32 rounding of 800 steps (1.05e-5 measured), <= 3e-5.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def shots():
+    import sgfd
+    from sgfd import configs
+
+    sgfd.lib()
+    _say("building the 32 shots")
+    return configs.batched_shots()
+
+
+def test_batched32_bit_identical_to_single_shots(shots):
+    import sgfd
+
+    nt = 100
+    solvers = [wl.solver() for wl in shots]
+    b = None
+    try:
+        b = sgfd.Batch(solvers)
+        _say(f"batch of {len(shots)} shots, {sum(w.points for w in shots) / 1e6:.1f} M points, {nt} steps")
+        b.step(nt)
+        b.synchronize()
+        for i, wl in enumerate(shots):
+            trb, Fb = solvers[i].stations(), solvers[i].fields()
+            s = wl.solver()
+            try:
+                s.step(nt)
+                s.synchronize()
+                tr1, F1 = s.stations(), s.fields()
+            finally:
+                s.close()
+            assert trb.shape == (nt, len(wl.stations), 3)
+            assert np.array_equal(trb, tr1), (i, wl.description)
+            assert np.array_equal(Fb, F1), (i, wl.description)
+            # the wave has left the source (surface receivers of a deep
+            # source may still be silent after 100 steps)
+            assert np.abs(Fb[0]).max() > 0, i
+    finally:
+        if b is not None:
+            b.close()
+        for s in solvers:
+            s.close()
+
+
+def test_batched32_largest_shots_fp32_against_fp64_full_run(shots):
+    order = sorted(range(len(shots)), key=lambda i: -shots[i].points)
+    for i in order[:2]:
+        wl = shots[i]
+        out = {}
+        for prec in ("fp32", "fp64"):
+            _say(f"shot {i} ({wl.description}) {prec}, {wl.nt} steps")
+            s = wl.solver(precision=prec)
+            try:
+                s.run()
+                out[prec] = (s.stations(), s.fields(np.float64))
+            finally:
+                s.close()
+        e_tr = _rel(out["fp32"][0], out["fp64"][0])
+        e_f = _rel(out["fp32"][1], out["fp64"][1])
+        _say(f"shot {i}: receivers {e_tr:.2e}, fields {e_f:.2e}")
+        # receivers at the parity bar; the final fields carry the FP32
+        # rounding of 800 steps (measured 1.05e-5 on the largest shot)
+        assert e_tr < 1e-5 and e_f < 3e-5, (i, e_tr, e_f)

@@ -2,13 +2,17 @@
 
 * 256^3 random medium (P16): the first 40 steps against the C oracle (its
   OpenMP build finishes them in seconds): every wavefield rel-L2 <= 1e-5.
+* 256^3 random medium, the full 1000 steps: FP32 against the FP64 build on
+  the 64x64 receivers, rel-L2 <= 1e-5, and on every wavefield, <= 3e-5
+  (measured 6.7e-6 to 1.2e-5: the FP32 rounding of 1000 steps).
 * 512x512x384 four-layer model (P20), the full 2000 steps: FP32 station
-  traces against the FP64 build of the same kernels, rel-L2 <= 1e-5; FP16
-  and BF16 storage (scales from an FP32 pilot run) within 1e-2 and 8e-2,
-  their measured levels being 7.6e-3 and 6.2e-2.
-* 1024x1024x640 four-layer model (P24), the full 1500 steps: FP32 against
-  the FP64 build on the 256x256 surface receivers and on a seeded 2 %
-  subsample of the points of vz and sxx, rel-L2 <= 1e-5.
+  traces and final wavefields against the FP64 build of the same kernels,
+  rel-L2 <= 1e-5; FP16 and BF16 storage (scales from an FP32 pilot run)
+  held below bounds above their measured levels (seismograms 7.6e-3 and
+  6.2e-2).
+* 1024x1024x640 four-layer model (P24), the full 1500 steps: FP32 and FP16
+  storage against the FP64 build on the 256x256 surface receivers and on a
+  seeded 2 % subsample of the points of vz and sxx, FP32 rel-L2 <= 1e-5.
 * 1344x1344x1300 (2.35 G points, beyond 2^31 per array), homogeneous,
   source 100 cells below the top: after 20 steps the wave has spread at most
   200 cells, so a 440x440x330 grid with the source at the same distance from
@@ -76,35 +80,76 @@ def test_random256_first_steps_against_oracle(mods, oracle_mod):
         assert e < 1e-5, (sgfd.FIELDS[f], e)
 
 
+def test_random256_fp32_against_fp64_full_run(mods):
+    sgfd, configs = mods
+    wl = configs.random256()
+    res = {}
+    for prec in ("fp64", "fp32"):
+        _say(f"random256 {wl.nt} steps {prec}")
+        s = wl.solver(prec)
+        try:
+            s.run()
+            res[prec] = (s.stations().astype(np.float64), s.fields(np.float64))
+        finally:
+            s.close()
+    assert res["fp64"][0].shape == (wl.nt, 64 * 64, 3)
+    e_tr = _rel(res["fp32"][0], res["fp64"][0])
+    _say(f"random256 receivers rel-L2 fp32 vs fp64: {e_tr:.3e}")
+    assert e_tr < 1e-5
+    # the final fields hold the FP32 rounding of 1000 steps of the scheme
+    # itself (measured 6.7e-6 to 1.2e-5 per field), which any FP32
+    # implementation carries; the parity bar against the FP32 oracle is the
+    # first-steps test above
+    for f in range(9):
+        e = _rel(res["fp32"][1][f], res["fp64"][1][f])
+        _say(f"random256 {sgfd.FIELDS[f]} rel-L2 fp32 vs fp64: {e:.3e}")
+        assert e < 3e-5, sgfd.FIELDS[f]
+
+
 def test_layered512_precisions_against_fp64_full_run(mods):
     sgfd, configs = mods
     wl = configs.layered512()
-    res = {}
+    ref_tr = ref_f = None
+    err, ferr = {}, {}
     for prec in ("fp64", "fp32", "fp16", "bf16"):
         _say(f"layered512 2000 steps {prec}")
         s = wl.solver(prec)
         try:
             s.step(wl.nt)
             s.synchronize()
-            res[prec] = s.stations().astype(np.float64)
+            tr = s.stations().astype(np.float64)
+            if prec == "fp64":
+                ref_tr, ref_f = tr, s.fields(np.float64)
+                continue
+            err[prec] = _rel(tr, ref_tr)
+            # final wavefields, all nine together, against the FP64 run
+            num = den = 0.0
+            for f in range(9):
+                d = s.field(f, np.float64)
+                num += float(np.sum((d - ref_f[f]) ** 2))
+                den += float(np.sum(ref_f[f] ** 2))
+            ferr[prec] = float(np.sqrt(num / den))
         finally:
             s.close()
-    err = {p: _rel(res[p], res["fp64"]) for p in ("fp32", "fp16", "bf16")}
     _say("seismogram rel-L2 against fp64: " + ", ".join(f"{p} {e:.3e}" for p, e in err.items()))
-    assert err["fp32"] < 1e-5
-    # 16-bit storage at its measured levels (7.6e-3 and 6.2e-2) with margin
-    assert 1e-3 < err["fp16"] < 1e-2
-    assert 1e-2 < err["bf16"] < 8e-2
+    _say("final-field rel-L2 against fp64: " + ", ".join(f"{p} {e:.3e}" for p, e in ferr.items()))
+    assert err["fp32"] < 1e-5 and ferr["fp32"] < 1e-5
+    # 16-bit storage: held to its measured levels with margin (seismograms
+    # 7.6e-3 / 6.2e-2 for FP16 / BF16); only upper bounds, so a more
+    # accurate 16-bit mode passes
+    # (measured: fields FP16 2.2e-2, BF16 1.8e-1)
+    assert err["fp16"] < 1e-2 and ferr["fp16"] < 3e-2
+    assert err["bf16"] < 8e-2 and ferr["bf16"] < 2.5e-1
 
 
-def test_large1024_fp32_against_fp64_subsample(mods):
+def test_large1024_fp32_fp16_against_fp64_subsample(mods):
     sgfd, configs = mods
     _say("large1024: building the model")
     wl = configs.large1024()
     n = wl.nx * wl.ny * wl.nz
     idx = np.random.default_rng(1024).choice(n, size=n // 50, replace=False)
     res = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp16"):
         _say(f"large1024 1500 steps {prec}")
         s = wl.solver(prec)
         try:
@@ -113,10 +158,14 @@ def test_large1024_fp32_against_fp64_subsample(mods):
             res[prec] = [s.stations().astype(np.float64)] + [s.field(f, np.float64).ravel()[idx] for f in ("vz", "sxx")]
         finally:
             s.close()
-    for what, a, b in zip(("stations", "vz 2 %", "sxx 2 %"), res["fp32"], res["fp64"]):
-        e = _rel(a, b)
-        _say(f"large1024 {what} rel-L2 fp32 vs fp64: {e:.3e}")
-        assert e < 1e-5, what
+    # FP32 at the parity bar; FP16 storage held to its measured level with
+    # margin (upper bound only)
+    # (FP16 measured: stations 6.7e-3, vz 9.7e-3, sxx 8.1e-3)
+    for prec, tol in (("fp32", 1e-5), ("fp16", 2e-2)):
+        for what, a, b in zip(("stations", "vz 2 %", "sxx 2 %"), res[prec], res["fp64"]):
+            e = _rel(a, b)
+            _say(f"large1024 {what} rel-L2 {prec} vs fp64: {e:.3e}")
+            assert e < tol, (prec, what)
 
 
 def test_layered512_linearity(mods):
