@@ -136,11 +136,15 @@ def _reconfigure(sgfd, s, wl, k):
     s.set_damping_local(*[sgfd.damping_profile(n, wl.pml, 10.0, "cuda", v=1.0, tau=4.3) for n in (wl.nx, wl.ny, wl.nz)])
 
 
+@pytest.mark.parametrize("wait", [True, False])
 @pytest.mark.parametrize("graph", [True, False])
-def test_batch_picks_up_shot_changes(sgfd_mod, graph):
+def test_batch_picks_up_shot_changes(sgfd_mod, graph, wait):
     """Shots reconfigured between batch steps (stations, source, damping)
     give the same results, bit for bit, as the same shots stepped alone with
-    the same changes at the same step."""
+    the same changes at the same step.  Without `wait` the setters are called
+    while the batched steps may still run: each setter waits for the batch
+    streams of the batches holding its solver, so nothing changes under a
+    running sweep."""
     sgfd = sgfd_mod
     wls = [_small_workload(sgfd, 30 + i, *d) for i, d in enumerate([(96, 80, 60), (70, 101, 50)])]
     half = wls[0].nt // 2
@@ -161,7 +165,8 @@ def test_batch_picks_up_shot_changes(sgfd_mod, graph):
         b = sgfd.Batch(solvers)
         b.set_launch(graph=graph)
         b.step(half)
-        b.synchronize()
+        if wait:
+            b.synchronize()
         for k, (s, wl) in enumerate(zip(solvers, wls)):
             _reconfigure(sgfd, s, wl, k)
         b.step(wls[0].nt - half)
@@ -261,3 +266,32 @@ def test_random_batches_match_single_shots(sgfd_mod, seed):
             b.close()
         for s in solvers:
             s.close()
+
+
+def test_failed_station_request_leaves_the_solver_unchanged(sgfd_mod):
+    """A station set whose buffer does not fit the device (3000 stations x 9
+    components x 2^26 steps, 7.2 TB) fails with SG_ERR_NOMEM, and the solver
+    keeps its previous stations: it steps and records exactly like a solver
+    that never saw the request."""
+    sgfd = sgfd_mod
+    wl = _small_workload(sgfd, 5, 64, 64, 64)
+    st = np.array([[30, 30, 30], [34, 30, 30]], np.int32)
+
+    def make(time_length):
+        s = sgfd.Solver(64, 64, 64, wl.pml, wl.dt, wl.dx, wl.dy, wl.dz, coef=wl.coef, time_length=time_length)
+        s.set_medium(wl.b, wl.lam, wl.mu)
+        s.set_source(*wl.src, wl.wavelet)
+        s.set_stations(st, comps=("vx", "sxx"))
+        return s
+
+    _say("failed station request, 64^3")
+    with make(wl.nt) as r:
+        r.step(wl.nt)
+        ref = (r.stations(), r.fields())
+    with make(1 << 26) as s:  # a 512 MiB station buffer
+        with pytest.raises(sgfd.SgError) as e:
+            s.set_stations(np.tile(st[:1], (3000, 1)), comps=sgfd.FIELDS)
+        assert e.value.code == -3  # SG_ERR_NOMEM
+        s.step(wl.nt)
+        assert np.array_equal(s.stations(), ref[0])
+        assert np.array_equal(s.fields(), ref[1])
