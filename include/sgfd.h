@@ -242,6 +242,16 @@ int sg_energy(sg_solver *s, double out[2]);
  * order (per z plane, then over the planes): the result does not depend on
  * scheduling. */
 int sg_energy_full(sg_solver *s, double out[3]);
+/* sg_energy without a library-allocated work buffer: the reduction's partial
+ * sums live in a fixed device array of the library (512 KiB per GPU, held
+ * until the process ends), so the call allocates nothing.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc).  Same result, bit
+ * for bit, as sg_energy.  Limit: grids of at most 16384 updated z planes
+ * (nz - 10); larger grids get SG_ERR_UNSUPPORTED.  All calls on one GPU
+ * share the array: calls that overlap on one GPU (from several threads)
+ * race and must be serialised by the caller. */
+int sg_energy_noalloc(sg_solver *s, double out[2]);
 /* Largest |value| of each wavefield (vx vy vz sxx syy szz sxy sxz syz),
  * unscaled.  Run an FP32 solver for a pilot interval and read this to pick
  * the power-of-two storage scales of an FP16/BF16 solver

@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -453,6 +454,35 @@ int check_solver(const sg_solver *s)
     return SG_OK;
 }
 
+// Device scratch of sg_energy_noalloc: the per-plane partial sums and the
+// results of the energy reduction, for grids of up to NOALLOC_MAX_PLANES
+// updated z planes (512 KiB).  Each GPU holds its own copy for the life of
+// the process; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch).
+constexpr int NOALLOC_MAX_PLANES = 16384;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[ENERGY_PART * NOALLOC_MAX_PLANES + 4];
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The scratch address on `device` (current): looked up once per device
+// (sg_create does it for every device it creates a solver on).
+int noalloc_scratch(int device, double **out)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return fail(SG_ERR_UNSUPPORTED, "device index beyond the scratch table (64 devices)");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return SG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -528,6 +558,11 @@ int sg_create(const sg_config *cfg, sg_solver **out)
     } while (0)
 
     SG_TRY(hipGetDevice(&s->device));
+    {
+        // the scratch address of sg_energy_noalloc on this device
+        double *scr = nullptr;
+        if ((rc = noalloc_scratch(s->device, &scr)) != SG_OK) return bail(rc);
+    }
     SG_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (int q = 0; q < 3; ++q) {
         SG_TRY(hipStreamCreateWithFlags(&s->side[q], hipStreamNonBlocking));
@@ -1038,6 +1073,26 @@ int sg_energy_full(sg_solver *s, double out[3])
     SG_HIP(launch_energy(s->prec, s->shot, s->d_energy, res, s->stream));
     SG_HIP(hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     SG_QUIESCE(s);
+    return SG_OK;
+}
+
+int sg_energy_noalloc(sg_solver *s, double out[2])
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!out) return fail(SG_ERR_INVALID, "sg_energy_noalloc: out is NULL");
+    if (s->g.uz > NOALLOC_MAX_PLANES)
+        return fail(SG_ERR_UNSUPPORTED, "sg_energy_noalloc: the grid has " + std::to_string(s->g.uz) +
+                                            " updated z planes; the scratch holds " +
+                                            std::to_string(NOALLOC_MAX_PLANES) + " (use sg_energy)");
+    SG_HIP(hipSetDevice(s->device));
+    double *scr = nullptr;
+    if (int rc = noalloc_scratch(s->device, &scr)) return rc;
+    SG_QUIESCE(s);
+    // every partial and result is written before it is read: nothing to clear
+    double *res = scr + (size_t)ENERGY_PART * s->g.uz;
+    SG_HIP(launch_energy(s->prec, s->shot, scr, res, s->stream));
+    SG_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    SG_HIP(hipStreamSynchronize(s->stream));
     return SG_OK;
 }
 

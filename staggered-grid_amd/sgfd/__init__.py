@@ -100,6 +100,7 @@ API = {
     "sg_get_field_f64": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     "sg_energy": (ctypes.c_int, [_VP, _DP]),
     "sg_energy_full": (ctypes.c_int, [_VP, _DP]),
+    "sg_energy_noalloc": (ctypes.c_int, [_VP, _DP]),
     "sg_courant": (ctypes.c_int, [_VP, _DP]),
     "sg_courant_number": (ctypes.c_int, [ctypes.c_longlong, _FP, _FP, _FP, ctypes.c_float, ctypes.c_float,
                                          ctypes.c_float, ctypes.c_float, _FP, _DP]),
