@@ -30,6 +30,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sg_internal.hpp"
 #include "sg_precision.hpp"
 
@@ -1028,6 +1030,71 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     }
 }
 
+}  // namespace sg
+
+#include "sg_tiled_lead.hpp"
+
+namespace sg {
+
+// Lead-plane staging (sg_tiled_lead.hpp) or staging at the updated plane,
+// per sweep, storage size and work-list: bit l of a mask selects the lead
+// form for the list with damping mask LIST_DM[l] (0, 1, 2, 4, 7).  Off by
+// default: measured at 512x512x384 FP32 (interior lists, 4 waves per SIMD,
+// lists one after another) the lead form reads 24 % (stress) and 17 %
+// (velocity) fewer bytes but takes 1.063 / 0.914 ms against 1.035 /
+// 0.846 ms: the plain sweeps run at the copy rate on their real traffic and
+// the lead sweeps hit a latency limit of about the same size.
+#ifndef SG_LEAD_V32
+#define SG_LEAD_V32 0
+#endif
+#ifndef SG_LEAD_S32
+#define SG_LEAD_S32 0
+#endif
+#ifndef SG_LEAD_V16
+#define SG_LEAD_V16 0
+#endif
+#ifndef SG_LEAD_S16
+#define SG_LEAD_S16 0
+#endif
+// (boxes of 4 rows, one row per thread only: the LDS ring holds one entry
+// per thread and plane)
+template <class T, int DM, int W, int TY, int NY>
+struct LeadOn {
+    static constexpr int bit = DM == 0 ? 0 : (DM == 1 ? 1 : (DM == 2 ? 2 : (DM == 4 ? 3 : 4)));
+    static constexpr int mask =
+        sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16) : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
+    static constexpr bool v = ((mask >> bit) & 1) != 0 && TY == 4 && NY == 1;
+};
+
+// the LDS of a velocity / stress box for the chosen form
+template <class T, class C, int TY, int NY, int DM>
+struct VelLdsOf {
+    using type = typename std::conditional<LeadOn<T, DM, 0, TY, NY>::v, VelLdsL<C, TY>, VelLds<C, TY>>::type;
+};
+template <class T, class C, int TY, int NY, int DM>
+struct StrLdsOf {
+    using type = typename std::conditional<LeadOn<T, DM, 1, TY, NY>::v, StrLdsL<C, TY, DM == 0 ? 5 : 6>,
+                                           StrLds<C, TY>>::type;
+};
+
+template <class T, class C, int TY, int NY, int DM, bool CP, class LT>
+__device__ __forceinline__ void vel_tile(const Shot<T, C> &s, const WorkItem &w, LT &L)
+{
+    if constexpr (LeadOn<T, DM, 0, TY, NY>::v)
+        velocity_tile_ld<T, C, TY, NY, DM, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+    else
+        velocity_tile<T, C, TY, NY, DM, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+}
+
+template <class T, class C, int TY, int NY, int DM, bool CP, class LT>
+__device__ __forceinline__ void str_tile(const Shot<T, C> &s, const WorkItem &w, LT &L)
+{
+    if constexpr (LeadOn<T, DM, 1, TY, NY>::v)
+        stress_tile_ld<T, C, TY, NY, DM, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+    else
+        stress_tile<T, C, TY, NY, DM, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+}
+
 // ---------------------------------------------------------------------------
 // kernels: a grid-stride loop over a work-list (one or many shots)
 // ---------------------------------------------------------------------------
@@ -1076,10 +1143,10 @@ template <class T, class C, int TY, int NY, int DM, bool CP>
 __global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, DM, NY, 0, CP>::v))) void k_velocity_wl(const Shot<T, C> *__restrict__ shots,
                                                             const WorkItem *__restrict__ items, int nitems)
 {
-    __shared__ VelLds<C, TY> L;
+    __shared__ typename VelLdsOf<T, C, TY, NY, DM>::type L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) velocity_tile<T, C, TY, NY, DM, CP>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        if (w.shot >= 0) vel_tile<T, C, TY, NY, DM, CP>(shots[w.shot], w, L);
         __syncthreads();
     }
 }
@@ -1088,10 +1155,10 @@ template <class T, class C, int TY, int NY, int DM, bool CP>
 __global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, DM, NY, 1, CP>::v))) void k_stress_wl(const Shot<T, C> *__restrict__ shots,
                                                           const WorkItem *__restrict__ items, int nitems)
 {
-    __shared__ StrLds<C, TY> L;
+    __shared__ typename StrLdsOf<T, C, TY, NY, DM>::type L;
     for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
         const WorkItem w = items[p];
-        if (w.shot >= 0) stress_tile<T, C, TY, NY, DM, CP>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        if (w.shot >= 0) str_tile<T, C, TY, NY, DM, CP>(shots[w.shot], w, L);
         __syncthreads();
     }
 }
@@ -1172,17 +1239,17 @@ template <class T, class C, bool CP>
 __global__ __launch_bounds__(TX * 4) void k_velocity_fused(const Shot<T, C> *__restrict__ shots,
                                                            const WorkItem *__restrict__ items, ListOffsets lo)
 {
-    __shared__ VelLds<C, 4> L;
+    __shared__ VelLdsL<C, 4> L;
     for (int p = blockIdx.x; p < lo.off[NLIST]; p += gridDim.x) {
         const WorkItem w = items[p];
         if (w.shot >= 0) {
             const Shot<T, C> &s = shots[w.shot];
             switch (list_of_item(lo, p)) {
-            case 0: velocity_tile<T, C, 4, 1, 0, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            case 1: velocity_tile<T, C, 4, 1, 1, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            case 2: velocity_tile<T, C, 4, 1, 2, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            case 3: velocity_tile<T, C, 4, 1, 4, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            default: velocity_tile<T, C, 4, 1, 7, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 0: vel_tile<T, C, 4, 1, 0, CP>(s, w, L); break;
+            case 1: vel_tile<T, C, 4, 1, 1, CP>(s, w, L); break;
+            case 2: vel_tile<T, C, 4, 1, 2, CP>(s, w, L); break;
+            case 3: vel_tile<T, C, 4, 1, 4, CP>(s, w, L); break;
+            default: vel_tile<T, C, 4, 1, 7, CP>(s, w, L); break;
             }
         }
         __syncthreads();
@@ -1193,17 +1260,17 @@ template <class T, class C, bool CP>
 __global__ __launch_bounds__(TX * 4) void k_stress_fused(const Shot<T, C> *__restrict__ shots,
                                                          const WorkItem *__restrict__ items, ListOffsets lo)
 {
-    __shared__ StrLds<C, 4> L;
+    __shared__ StrLdsL<C, 4, 6> L;
     for (int p = blockIdx.x; p < lo.off[NLIST]; p += gridDim.x) {
         const WorkItem w = items[p];
         if (w.shot >= 0) {
             const Shot<T, C> &s = shots[w.shot];
             switch (list_of_item(lo, p)) {
-            case 0: stress_tile<T, C, 4, 1, 0, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            case 1: stress_tile<T, C, 4, 1, 1, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            case 2: stress_tile<T, C, 4, 1, 2, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            case 3: stress_tile<T, C, 4, 1, 4, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
-            default: stress_tile<T, C, 4, 1, 7, CP>(s, w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L); break;
+            case 0: str_tile<T, C, 4, 1, 0, CP>(s, w, L); break;
+            case 1: str_tile<T, C, 4, 1, 1, CP>(s, w, L); break;
+            case 2: str_tile<T, C, 4, 1, 2, CP>(s, w, L); break;
+            case 3: str_tile<T, C, 4, 1, 4, CP>(s, w, L); break;
+            default: str_tile<T, C, 4, 1, 7, CP>(s, w, L); break;
             }
         }
         __syncthreads();
