@@ -118,10 +118,18 @@ def bench_batched(args):
     log(f"bench: shots ready in {time.perf_counter() - t:.1f} s")
     solvers = [wl.solver() for wl in shots]
     b = None
+    tuned = None
     try:
         b = sgfd.Batch(solvers)
         if args.tile:
             b.set_launch(*args.tile)
+        elif not args.no_tune:
+            # z-chunk, register bound and grid of the batched launch, timed
+            # outside the measured steps (resets every shot)
+            log("bench: batch autotune")
+            tile, grid, tms = b.autotune(3)
+            tuned = {"tile": list(tile), "grid": grid, "ms_per_step": tms}
+            log(f"bench: autotune chose tile {tile} grid {grid} ({tms:.2f} ms per step)")
         b.step(args.warmup)
         b.synchronize()
         ms = b.step_timed(args.steps)
@@ -162,6 +170,7 @@ def bench_batched(args):
                         "achieved": (vb + sb) / (ms / K * 1e-3) / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": step_frac, "traffic": None},
            "sequential_value": seq_value,
+           "launch": tuned,
            "cpu_baseline": None}
     print(json.dumps(out))
 
@@ -273,6 +282,7 @@ def main():
                          "z planes per box (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage leg of an FP32 run")
+    ap.add_argument("--no-tune", action="store_true", help="batched32: keep the default launch (no autotuner)")
     ap.add_argument("--streams", type=int, default=None, help="streams per sweep (1, 2, 3)")
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
     ap.add_argument("--compare-sequential", action="store_true",

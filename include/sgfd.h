@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SG_API_VERSION 1
+#define SG_API_VERSION 2
 
 typedef struct sg_solver sg_solver;
 typedef struct sg_batch sg_batch;
@@ -270,13 +270,18 @@ int sg_set_graph(sg_solver *s, int enable);
  * and z_chunk planes deep; a block of 64 x tile_y/rows_per_thread threads
  * computes it.  Interior boxes (no damped point) and PML boxes have their
  * own shape.  (tile_y, rows_per_thread) is one of (4,1) (8,1) (8,2) (16,1)
- * (16,2); z_chunk 0 = automatic.  The shape never changes a result. */
+ * (16,2); z_chunk 0 = automatic.  waves_per_simd bounds the registers of
+ * the one-launch-per-sweep form (the five work-lists in one kernel, used by
+ * a one-stream solver with (4,1) boxes throughout and by batches with (4,1)
+ * boxes), so that at least that many waves fit on a SIMD: 0 = no bound, or
+ * 2, 3, 4 (FP32 storage only).  Neither shape nor bound changes a result. */
 typedef struct sg_tile {
     int tile_y;
     int rows_per_thread;
     int pml_tile_y;
     int pml_rows_per_thread;
     int z_chunk;
+    int waves_per_simd;
 } sg_tile;
 int sg_set_tile(sg_solver *s, const sg_tile *tile);
 int sg_get_tile(const sg_solver *s, sg_tile *tile);
@@ -288,9 +293,11 @@ int sg_get_tile(const sg_solver *s, sg_tile *tile);
  * points, so results do not change. */
 int sg_set_streams(sg_solver *s, int nstreams);
 /* Autotuner: times `steps` steps for a set of interior shapes, PML shapes
- * and z-chunk lengths, keeps the fastest (written to *best; per-step sweep
- * time in *best_ms) and RESETS the wavefields (sg_reset): call it before the
- * real run. */
+ * and z-chunk lengths, then (FP32 storage) the one-launch-per-sweep form on
+ * one stream with each register bound (waves_per_simd 0, 2, 3, 4); keeps the
+ * fastest (written to *best, together with the stream count it needs, set
+ * as by sg_set_streams; per-step sweep time in *best_ms) and RESETS the
+ * wavefields (sg_reset): call it before the real run. */
 int sg_autotune(sg_solver *s, int steps, sg_tile *best, double *best_ms);
 /* Algorithmic HBM bytes of one velocity and one stress sweep (the lean
  * split-PML word count times the storage element size). */
@@ -312,8 +319,16 @@ int sg_batch_create(sg_solver *const *shots, int n, sg_batch **out);
 /* Launch shape of the batched sweeps: box shapes (as sg_set_tile; z_chunk
  * must be > 0), grid size (0 = one block per work item; smaller = persistent
  * blocks striding over the list) and whether one step is replayed from a
- * hipGraph. */
+ * hipGraph.  With (4,1) boxes throughout each half-step is ONE launch over
+ * the work-lists of all shots, its registers bounded by waves_per_simd;
+ * other shapes run one launch per work-list (the default: (16,1) interior
+ * and (4,1) PML boxes, 24-plane chunks, one block per item, graph on). */
 int sg_batch_set_launch(sg_batch *b, const sg_tile *tile, int grid, int use_graph);
+/* Batch autotuner: times `steps` batched steps for z-chunk lengths, register
+ * bounds and grid sizes of the one-launch form (and two per-list shapes),
+ * keeps the fastest (its tile and grid written to *best and *best_grid, the
+ * time per step to *best_ms) and RESETS every shot (sg_reset). */
+int sg_batch_autotune(sg_batch *b, int steps, sg_tile *best, int *best_grid, double *best_ms);
 int sg_batch_step(sg_batch *b, int nsteps);
 /* sg_batch_step bracketed by HIP events on the batch stream; waits. */
 int sg_batch_step_timed(sg_batch *b, int nsteps, double *ms);

@@ -1,10 +1,17 @@
 // sg_batch.cpp -- many independent shots advanced together.
 //
-// All shots' tiles form one work-list; each half-step is ONE launch of a
-// grid-stride (persistent) kernel over it, so small and uneven shots share
-// the GPU instead of running one under-filled launch each.  Source injection
-// and station recording of all shots are one launch each.  A step can be
-// captured in a hipGraph and replayed.
+// The boxes of all shots form one work-list per damping mask (interior, x,
+// y, z faces, edges and corners), stored back to back, so small and uneven
+// shots share the GPU instead of running one under-filled launch each.
+// With (4,1) boxes throughout each half-step is ONE launch over all of them:
+// a grid-stride kernel whose blocks pick each item's damping-mask code, its
+// registers bounded to a chosen number of waves per SIMD, the grid one block
+// per item or a persistent grid.  Other box shapes run one launch per list
+// (the default, 16-row interior boxes: measured faster at the 32-shot size,
+// since one launch carries the registers of the edge and corner code), the
+// PML lists on two side streams.  Source injection and station recording of
+// all shots are one launch each.  A step is captured in a hipGraph and
+// replayed; sg_batch_autotune picks the form, z-chunk, bound and grid.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,7 +52,7 @@ struct sg_batch {
     WorkItem *d_items = nullptr;
     int off[NLIST + 1] = {0, 0, 0, 0, 0, 0};  // work-lists back to back (each padded to a multiple of 8)
     int grid = 0;
-    Tile tile{4, 1, 4, 1, 24};
+    Tile tile{16, 1, 4, 1, 24, 0};
     int max_gather = 0;
     // the shot and auxiliary descriptors as last uploaded to d_shots/d_aux
     std::vector<unsigned char> shot_host;
@@ -119,6 +126,14 @@ int enqueue(sg_batch *b)
     const int n = (int)b->shots.size();
     SG_HIP(launch_inject_batch(b->d_shots, b->d_aux, n, b->stream));
     const Tile &t = b->tile;
+    if (tile_fusable(t)) {
+        // one launch per half-step over every shot's work-lists
+        for (int which = 0; which < 2; ++which)
+            SG_HIP(launch_sweep_fused(which, SG_FP32, 0, t.waves, b->d_shots, b->d_items, b->off, b->grid,
+                                      b->stream));
+        SG_HIP(launch_gather_batch(b->d_shots, b->d_aux, n, b->max_gather, b->stream));
+        return SG_OK;
+    }
     for (int which = 0; which < 2; ++which) {
         SG_HIP(hipEventRecord(b->fork, b->stream));
         for (int q = 0; q < 2; ++q) SG_HIP(hipStreamWaitEvent(b->side[q], b->fork, 0));
@@ -212,9 +227,12 @@ int sg_batch_set_launch(sg_batch *b, const sg_tile *tile, int grid, int use_grap
         return fail(SG_ERR_INVALID,
                     "sg_batch_set_launch: (tile_y, rows_per_thread) must be (4,1), (8,1), (8,2), (16,1) or (16,2)");
     if (tile->z_chunk < 1 || grid < 0) return fail(SG_ERR_INVALID, "sg_batch_set_launch: bad z_chunk or grid");
+    if (!waves_ok(tile->waves_per_simd))
+        return fail(SG_ERR_INVALID, "sg_batch_set_launch: waves_per_simd must be 0, 2, 3 or 4");
     SG_HIP(hipSetDevice(b->device));
     SG_HIP(hipStreamSynchronize(b->stream));
-    b->tile = Tile{tile->tile_y, tile->rows_per_thread, tile->pml_tile_y, tile->pml_rows_per_thread, tile->z_chunk};
+    b->tile = Tile{tile->tile_y, tile->rows_per_thread, tile->pml_tile_y, tile->pml_rows_per_thread, tile->z_chunk,
+                   tile->waves_per_simd};
     b->grid = grid;
     b->use_graph = use_graph != 0;
     drop_graph(b);
@@ -271,6 +289,59 @@ int sg_batch_step_timed(sg_batch *b, int nsteps, double *ms)
     (void)hipEventDestroy(e1);
     if (ms) *ms = t;
     return rc;
+}
+
+int sg_batch_autotune(sg_batch *b, int steps, sg_tile *best_tile, int *best_grid, double *best_ms)
+{
+    if (!b) return fail(SG_ERR_INVALID, "sg_batch_autotune: NULL batch");
+    if (steps < 1) return fail(SG_ERR_INVALID, "sg_batch_autotune: steps < 1");
+    SG_HIP(hipSetDevice(b->device));
+    int cus = 0;
+    SG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    const Tile t0 = b->tile;
+    const int g0 = b->grid;
+    const bool graph0 = b->use_graph;
+    Tile bt = t0;
+    int bg = g0;
+    double best = 1e300;
+    auto trial = [&](const Tile &t, int grid) -> int {
+        const sg_tile st{t.ty, t.ny, t.pty, t.pny, t.zchunk, t.waves};
+        if (int rc = sg_batch_set_launch(b, &st, grid, graph0 ? 1 : 0)) return rc;
+        if (int rc = sg_batch_step(b, 1)) return rc;  // capture outside the timing
+        double ms = 0;
+        if (int rc = sg_batch_step_timed(b, steps, &ms)) return rc;
+        if (ms < best) {
+            best = ms;
+            bt = t;
+            bg = grid;
+        }
+        return SG_OK;
+    };
+    int rc = SG_OK;
+    // the one-launch form: z-chunk x register bound x grid (one block per
+    // item, or 2 or 4 persistent blocks per CU)
+    const int zcs[] = {16, 24, 32};
+    const int wvs[] = {0, 2, 3, 4};
+    const int grids[] = {0, 2 * cus, 4 * cus};
+    for (int zc : zcs)
+        for (int w : wvs)
+            for (int g : grids)
+                if (!rc) rc = trial(Tile{4, 1, 4, 1, zc, w}, g);
+    // one launch per list with taller interior boxes
+    if (!rc) rc = trial(Tile{8, 1, 4, 1, 24, 0}, 0);
+    if (!rc) rc = trial(Tile{16, 1, 4, 1, 24, 0}, 0);
+    const sg_tile bst{bt.ty, bt.ny, bt.pty, bt.pny, bt.zchunk, bt.waves};
+    const sg_tile st0{t0.ty, t0.ny, t0.pty, t0.pny, t0.zchunk, t0.waves};
+    const int rc2 = sg_batch_set_launch(b, rc ? &st0 : &bst, rc ? g0 : bg, graph0 ? 1 : 0);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    SG_HIP(hipStreamSynchronize(b->stream));
+    for (auto *s : b->shots)
+        if (int r = sg_reset(s)) return r;
+    if (best_tile) *best_tile = bst;
+    if (best_grid) *best_grid = bg;
+    if (best_ms) *best_ms = best / steps;
+    return SG_OK;
 }
 
 int sg_batch_synchronize(sg_batch *b)

@@ -122,7 +122,12 @@ struct Tile {
     int ty, ny;    // interior boxes: y rows per box, y rows per thread
     int pty, pny;  // PML boxes
     int zchunk;    // z planes per box (0 = automatic)
+    int waves;     // register bound of the one-launch form (waves per SIMD, 0 none)
 };
+
+// (4,1) boxes throughout: the five lists of a sweep can run as one launch
+inline bool tile_fusable(const Tile &t) { return t.ty == 4 && t.ny == 1 && t.pty == 4 && t.pny == 1; }
+inline bool waves_ok(int w) { return w == 0 || w == 2 || w == 3 || w == 4; }
 
 inline bool tile_shape_ok(int ty, int ny)
 {
@@ -176,16 +181,17 @@ hipError_t sweep_bf16(int which, int ty, int ny, int dm, int comp, const void *s
                       int grid, hipStream_t st);
 
 // all work-lists of a sweep in one launch, (4,1) boxes (off: NLIST+1 list
-// offsets into items)
-hipError_t sweep_fused_fp32(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+// offsets into items); waves: register bound of the launch in waves per
+// SIMD (0 none; 2, 3, 4 for FP32 storage)
+hipError_t sweep_fused_fp32(int which, int comp, int waves, const void *shots, const WorkItem *items, const int *off, int grid,
                             hipStream_t st);
-hipError_t sweep_fused_fp64(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+hipError_t sweep_fused_fp64(int which, int comp, int waves, const void *shots, const WorkItem *items, const int *off, int grid,
                             hipStream_t st);
-hipError_t sweep_fused_fp16(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+hipError_t sweep_fused_fp16(int which, int comp, int waves, const void *shots, const WorkItem *items, const int *off, int grid,
                             hipStream_t st);
-hipError_t sweep_fused_bf16(int which, int comp, const void *shots, const WorkItem *items, const int *off, int grid,
+hipError_t sweep_fused_bf16(int which, int comp, int waves, const void *shots, const WorkItem *items, const int *off, int grid,
                             hipStream_t st);
-hipError_t launch_sweep_fused(int which, int prec, int comp, const void *shots_dev, const WorkItem *items,
+hipError_t launch_sweep_fused(int which, int prec, int comp, int waves, const void *shots_dev, const WorkItem *items,
                               const int *off, int grid, hipStream_t st);
 
 // work-lists: interior, x face, y face, z face, edges/corners

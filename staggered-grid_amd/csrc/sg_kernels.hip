@@ -554,14 +554,14 @@ hipError_t launch_sweep(int which, int prec, int ty, int ny, int dm, int comp, c
     }
 }
 
-hipError_t launch_sweep_fused(int which, int prec, int comp, const void *shots_dev, const WorkItem *items,
+hipError_t launch_sweep_fused(int which, int prec, int comp, int waves, const void *shots_dev, const WorkItem *items,
                               const int *off, int grid, hipStream_t st)
 {
     switch (prec) {
-    case 0: return sweep_fused_fp32(which, comp, shots_dev, items, off, grid, st);
-    case 1: return sweep_fused_fp64(which, comp, shots_dev, items, off, grid, st);
-    case 2: return sweep_fused_fp16(which, comp, shots_dev, items, off, grid, st);
-    case 3: return sweep_fused_bf16(which, comp, shots_dev, items, off, grid, st);
+    case 0: return sweep_fused_fp32(which, comp, waves, shots_dev, items, off, grid, st);
+    case 1: return sweep_fused_fp64(which, comp, waves, shots_dev, items, off, grid, st);
+    case 2: return sweep_fused_fp16(which, comp, waves, shots_dev, items, off, grid, st);
+    case 3: return sweep_fused_bf16(which, comp, waves, shots_dev, items, off, grid, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -100,7 +100,7 @@ struct sg_solver {
     hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
     int nstreams = 3;
     Geom g{};
-    Tile tile{4, 1, 4, 1, 0};
+    Tile tile{4, 1, 4, 1, 0, 0};
 
     void *fields[NFIELD] = {};
     void *model[3] = {};
@@ -326,10 +326,11 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
         const int ns = s->nstreams;
         // one stream and (4,1) boxes throughout: the five lists of a sweep in
         // one launch (small grids are launch-bound)
-        const bool fused = ns == 1 && t.ty == 4 && t.ny == 1 && t.pty == 4 && t.pny == 1;
+        const bool fused = ns == 1 && tile_fusable(t);
         for (int which = 0; fused && which < 2; ++which) {
             if (which == 1 && e1) SG_HIP(hipEventRecord(e1, s->stream));
-            SG_HIP(launch_sweep_fused(which, s->prec, s->comp_mask != 0, s->d_shot, s->d_wl, s->wl_off, 0, s->stream));
+            SG_HIP(launch_sweep_fused(which, s->prec, s->comp_mask != 0, t.waves, s->d_shot, s->d_wl, s->wl_off, 0,
+                                      s->stream));
         }
         for (int which = 0; !fused && which < 2; ++which) {
             if (which == 1 && e1) SG_HIP(hipEventRecord(e1, s->stream));
@@ -1256,7 +1257,11 @@ int sg_set_tile(sg_solver *s, const sg_tile *tile)
         return fail(SG_ERR_INVALID,
                     "sg_set_tile: (tile_y, rows_per_thread) must be (4,1), (8,1), (8,2), (16,1) or (16,2)");
     if (tile->z_chunk < 0) return fail(SG_ERR_INVALID, "sg_set_tile: z_chunk must be >= 0");
-    s->tile = Tile{tile->tile_y, tile->rows_per_thread, tile->pml_tile_y, tile->pml_rows_per_thread, tile->z_chunk};
+    if (!waves_ok(tile->waves_per_simd)) return fail(SG_ERR_INVALID, "sg_set_tile: waves_per_simd must be 0, 2, 3 or 4");
+    if (tile->waves_per_simd != 0 && s->prec != SG_FP32)
+        return fail(SG_ERR_UNSUPPORTED, "sg_set_tile: waves_per_simd bounds exist for FP32 storage only");
+    s->tile = Tile{tile->tile_y, tile->rows_per_thread, tile->pml_tile_y, tile->pml_rows_per_thread, tile->z_chunk,
+                   tile->waves_per_simd};
     s->graph_valid = false;
     s->wl_valid = false;
     return SG_OK;
@@ -1266,7 +1271,7 @@ int sg_get_tile(const sg_solver *s, sg_tile *tile)
 {
     if (int rc = check_solver(s)) return rc;
     if (!tile) return fail(SG_ERR_INVALID, "sg_get_tile: NULL tile");
-    *tile = sg_tile{s->tile.ty, s->tile.ny, s->tile.pty, s->tile.pny, s->tile.zchunk};
+    *tile = sg_tile{s->tile.ty, s->tile.ny, s->tile.pty, s->tile.pny, s->tile.zchunk, s->tile.waves};
     return SG_OK;
 }
 
@@ -1275,24 +1280,29 @@ int sg_autotune(sg_solver *s, int steps, sg_tile *best_tile, double *best_ms)
     if (int rc = check_solver(s)) return rc;
     if (steps < 1) return fail(SG_ERR_INVALID, "sg_autotune: steps < 1");
     if (s->cfg.kernel != SG_KERNEL_TILED) return fail(SG_ERR_STATE, "sg_autotune: the solver uses the per-point kernels");
-    // Two passes, each candidate timed over `steps` steps from the current
+    // Three passes, each candidate timed over `steps` steps from the current
     // state: interior shape x z-chunk length with the PML shape fixed, then
-    // the PML shape.  The state is reset afterwards.
+    // the PML shape, then the one-launch form with each register bound.
+    // The state is reset afterwards.
     const int uz = s->g.uz;
     const int shapes[][2] = {{4, 1}, {8, 1}, {16, 1}, {8, 2}, {16, 2}};
     const int zcs[] = {0, 8, 12, 16, 24, 48};
     double best = 1e300;
     Tile bt = s->tile;
+    const int ns0 = s->nstreams;
+    int bns = ns0;
     const bool graph = s->use_graph;
     s->use_graph = false;
-    auto trial = [&](const Tile &t) -> int {
+    auto trial = [&](const Tile &t, int ns) -> int {
         s->tile = t;
+        s->nstreams = ns;
         s->wl_valid = false;
         double v = 0, st = 0, tot = 0;
         if (int rc = sg_step_timed(s, steps, &v, &st, &tot)) return rc;
         if (v + st < best) {
             best = v + st;
             bt = t;
+            bns = ns;
         }
         return SG_OK;
     };
@@ -1300,16 +1310,23 @@ int sg_autotune(sg_solver *s, int steps, sg_tile *best_tile, double *best_ms)
     const Tile start = s->tile;
     for (const auto &sh : shapes)
         for (int zc : zcs)
-            if (!rc && zc <= uz) rc = trial(Tile{sh[0], sh[1], start.pty, start.pny, zc});
+            if (!rc && zc <= uz) rc = trial(Tile{sh[0], sh[1], start.pty, start.pny, zc, start.waves}, ns0);
     const Tile mid = bt;
     for (const auto &sh : shapes)
-        if (!rc && (sh[0] != mid.pty || sh[1] != mid.pny)) rc = trial(Tile{mid.ty, mid.ny, sh[0], sh[1], mid.zchunk});
+        if (!rc && (sh[0] != mid.pty || sh[1] != mid.pny))
+            rc = trial(Tile{mid.ty, mid.ny, sh[0], sh[1], mid.zchunk, mid.waves}, ns0);
+    // the one-launch-per-sweep form on one stream, each register bound
+    // (separate instantiations; FP32 storage)
+    const int wvs[] = {0, 2, 3, 4};
+    for (int w : wvs)
+        if (!rc && (w == 0 || s->prec == SG_FP32) && s->comp_mask == 0) rc = trial(Tile{4, 1, 4, 1, bt.zchunk, w}, 1);
     s->use_graph = graph;
     s->tile = bt;
+    s->nstreams = bns;
     s->graph_valid = false;
     s->wl_valid = false;
     if (rc) return rc;
-    if (best_tile) *best_tile = sg_tile{bt.ty, bt.ny, bt.pty, bt.pny, bt.zchunk};
+    if (best_tile) *best_tile = sg_tile{bt.ty, bt.ny, bt.pty, bt.pny, bt.zchunk, bt.waves};
     if (best_ms) *best_ms = best / steps;
     return sg_reset(s);
 }

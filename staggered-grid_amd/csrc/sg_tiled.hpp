@@ -1235,11 +1235,24 @@ __device__ __forceinline__ int list_of_item(const ListOffsets &lo, int p)
     return l;
 }
 
-template <class T, class C, bool CP>
-__global__ __launch_bounds__(TX * 4) void k_velocity_fused(const Shot<T, C> *__restrict__ shots,
-                                                           const WorkItem *__restrict__ items, ListOffsets lo)
+// LDS of a fused launch: large enough for every list's form
+template <class T, class C, int W>
+struct FusedLds {
+    static constexpr int mask = sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16)
+                                               : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
+    using type = typename std::conditional<
+        W == 0, typename std::conditional<(mask & 0x1F) != 0, VelLdsL<C, 4>, VelLds<C, 4>>::type,
+        typename std::conditional<(mask & 0x1F) != 0, StrLdsL<C, 4, 6>, StrLds<C, 4>>::type>::type;
+};
+
+// WV: minimum waves per SIMD the register allocation must allow (0: no
+// bound).  One launch carries the registers of its heaviest list (edges and
+// corners), which otherwise caps the occupancy of every box.
+template <class T, class C, bool CP, int WV>
+__global__ __launch_bounds__(TX * 4) __attribute__((amdgpu_waves_per_eu(WV > 0 ? WV : 1))) void k_velocity_fused(
+    const Shot<T, C> *__restrict__ shots, const WorkItem *__restrict__ items, ListOffsets lo)
 {
-    __shared__ VelLdsL<C, 4> L;
+    __shared__ typename FusedLds<T, C, 0>::type L;
     for (int p = blockIdx.x; p < lo.off[NLIST]; p += gridDim.x) {
         const WorkItem w = items[p];
         if (w.shot >= 0) {
@@ -1256,11 +1269,11 @@ __global__ __launch_bounds__(TX * 4) void k_velocity_fused(const Shot<T, C> *__r
     }
 }
 
-template <class T, class C, bool CP>
-__global__ __launch_bounds__(TX * 4) void k_stress_fused(const Shot<T, C> *__restrict__ shots,
-                                                         const WorkItem *__restrict__ items, ListOffsets lo)
+template <class T, class C, bool CP, int WV>
+__global__ __launch_bounds__(TX * 4) __attribute__((amdgpu_waves_per_eu(WV > 0 ? WV : 1))) void k_stress_fused(
+    const Shot<T, C> *__restrict__ shots, const WorkItem *__restrict__ items, ListOffsets lo)
 {
-    __shared__ StrLdsL<C, 4, 6> L;
+    __shared__ typename FusedLds<T, C, 1>::type L;
     for (int p = blockIdx.x; p < lo.off[NLIST]; p += gridDim.x) {
         const WorkItem w = items[p];
         if (w.shot >= 0) {
@@ -1277,10 +1290,21 @@ __global__ __launch_bounds__(TX * 4) void k_stress_fused(const Shot<T, C> *__res
     }
 }
 
-// off: list offsets into items (off[NLIST] items in all)
-template <class T, class C>
-hipError_t sweep_fused_t(int which, bool comp, const void *shots, const WorkItem *items, const int *off, int grid,
+template <class T, class C, bool CP, int WV>
+static void launch_fused(int which, const Shot<T, C> *sh, const WorkItem *items, const ListOffsets &lo, int grid,
                          hipStream_t st)
+{
+    const dim3 block(TX, 4), g(grid);
+    if (which == 0) hipLaunchKernelGGL((k_velocity_fused<T, C, CP, WV>), g, block, 0, st, sh, items, lo);
+    else hipLaunchKernelGGL((k_stress_fused<T, C, CP, WV>), g, block, 0, st, sh, items, lo);
+}
+
+// off: list offsets into items (off[NLIST] items in all); waves: the
+// register bound of the launch (0 none; 2, 3 or 4 waves per SIMD, FP32
+// storage only)
+template <class T, class C>
+hipError_t sweep_fused_t(int which, bool comp, int waves, const void *shots, const WorkItem *items, const int *off,
+                         int grid, hipStream_t st)
 {
     ListOffsets lo;
     for (int l = 0; l <= NLIST; ++l) lo.off[l] = off[l];
@@ -1288,18 +1312,23 @@ hipError_t sweep_fused_t(int which, bool comp, const void *shots, const WorkItem
     if (n <= 0) return hipSuccess;
     if (grid <= 0 || grid > n) grid = n;
     const Shot<T, C> *sh = static_cast<const Shot<T, C> *>(shots);
-    const dim3 block(TX, 4), g(grid);
     if constexpr (sizeof(T) == 2) {
-        if (comp) {
-            if (which == 0) hipLaunchKernelGGL((k_velocity_fused<T, C, true>), g, block, 0, st, sh, items, lo);
-            else hipLaunchKernelGGL((k_stress_fused<T, C, true>), g, block, 0, st, sh, items, lo);
-            return hipGetLastError();
-        }
+        if (waves != 0) return hipErrorInvalidValue;
+        if (comp) launch_fused<T, C, true, 0>(which, sh, items, lo, grid, st);
+        else launch_fused<T, C, false, 0>(which, sh, items, lo, grid, st);
+    } else if constexpr (sizeof(T) == 8) {
+        if (comp || waves != 0) return hipErrorInvalidValue;
+        launch_fused<T, C, false, 0>(which, sh, items, lo, grid, st);
     } else {
         if (comp) return hipErrorInvalidValue;
+        switch (waves) {
+        case 0: launch_fused<T, C, false, 0>(which, sh, items, lo, grid, st); break;
+        case 2: launch_fused<T, C, false, 2>(which, sh, items, lo, grid, st); break;
+        case 3: launch_fused<T, C, false, 3>(which, sh, items, lo, grid, st); break;
+        case 4: launch_fused<T, C, false, 4>(which, sh, items, lo, grid, st); break;
+        default: return hipErrorInvalidValue;
+        }
     }
-    if (which == 0) hipLaunchKernelGGL((k_velocity_fused<T, C, false>), g, block, 0, st, sh, items, lo);
-    else hipLaunchKernelGGL((k_stress_fused<T, C, false>), g, block, 0, st, sh, items, lo);
     return hipGetLastError();
 }
 

@@ -9,10 +9,10 @@ hipError_t sweep_fp32(int which, int ty, int ny, int dm, int comp, const void *s
     return sweep_t<float, float>(which, ty, ny, dm, comp != 0, shots, items, nitems, grid, st);
 }
 
-hipError_t sweep_fused_fp32(int which, int comp, const void *shots, const WorkItem *items, const int *off,
+hipError_t sweep_fused_fp32(int which, int comp, int waves, const void *shots, const WorkItem *items, const int *off,
                          int grid, hipStream_t st)
 {
-    return sweep_fused_t<float, float>(which, comp != 0, shots, items, off, grid, st);
+    return sweep_fused_t<float, float>(which, comp != 0, waves, shots, items, off, grid, st);
 }
 
 }  // namespace sg

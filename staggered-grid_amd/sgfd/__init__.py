@@ -51,10 +51,11 @@ _lib = None
 _FP = ctypes.POINTER(ctypes.c_float)
 class _Tile(ctypes.Structure):
     _fields_ = [("tile_y", ctypes.c_int), ("rows_per_thread", ctypes.c_int), ("pml_tile_y", ctypes.c_int),
-                ("pml_rows_per_thread", ctypes.c_int), ("z_chunk", ctypes.c_int)]
+                ("pml_rows_per_thread", ctypes.c_int), ("z_chunk", ctypes.c_int), ("waves_per_simd", ctypes.c_int)]
 
     def as_tuple(self):
-        return (self.tile_y, self.rows_per_thread, self.pml_tile_y, self.pml_rows_per_thread, self.z_chunk)
+        return (self.tile_y, self.rows_per_thread, self.pml_tile_y, self.pml_rows_per_thread, self.z_chunk,
+                self.waves_per_simd)
 
 
 _TP = ctypes.POINTER(_Tile)
@@ -123,6 +124,7 @@ API = {
     "sg_batch_step": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_batch_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     "sg_batch_synchronize": (ctypes.c_int, [_VP]),
+    "sg_batch_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _TP, _IP, _DP]),
     "sg_batch_destroy": (ctypes.c_int, [_VP]),
     "sg_ricker": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, _FP]),
     "sg_damping_profile": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -385,10 +387,11 @@ class Solver:
     def set_graph(self, enable=True):
         _check(lib().sg_set_graph(self._h, 1 if enable else 0))
 
-    def set_tile(self, ty=4, ny=1, pty=4, pny=1, zchunk=0):
+    def set_tile(self, ty=4, ny=1, pty=4, pny=1, zchunk=0, waves=0):
         """Box shapes: interior (ty rows per box, ny rows per thread), PML
-        (pty, pny), zchunk planes per box (0 = auto)."""
-        _check(lib().sg_set_tile(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk))))
+        (pty, pny), zchunk planes per box (0 = auto); waves: register bound
+        of the one-launch-per-sweep form (0 none, 2, 3, 4)."""
+        _check(lib().sg_set_tile(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk, waves))))
 
     def tile(self):
         t = _Tile()
@@ -396,8 +399,9 @@ class Solver:
         return t.as_tuple()
 
     def autotune(self, steps=5):
-        """Pick the fastest box shapes (resets the wavefields); returns
-        ((ty, ny, pty, pny, zchunk), ms per step)."""
+        """Pick the fastest box shapes, register bound and stream count
+        (resets the wavefields); returns ((ty, ny, pty, pny, zchunk, waves),
+        ms per step)."""
         t, ms = _Tile(), ctypes.c_double()
         _check(lib().sg_autotune(self._h, steps, ctypes.byref(t), ctypes.byref(ms)))
         return t.as_tuple(), ms.value
@@ -505,9 +509,20 @@ class Batch:
         _check(lib().sg_batch_create(arr, len(self.solvers), ctypes.byref(h)))
         self._h = h
 
-    def set_launch(self, ty=4, ny=1, pty=4, pny=1, zchunk=24, grid=0, graph=True):
-        _check(lib().sg_batch_set_launch(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk)), grid,
+    def set_launch(self, ty=16, ny=1, pty=4, pny=1, zchunk=24, grid=0, graph=True, waves=0):
+        """Box shapes, z-chunk, grid (0: one block per item), graph replay
+        and the register bound; (4,1) boxes throughout run each half-step as
+        one launch."""
+        _check(lib().sg_batch_set_launch(self._h, ctypes.byref(_Tile(ty, ny, pty, pny, zchunk, waves)), grid,
                                          1 if graph else 0))
+
+    def autotune(self, steps=5):
+        """Pick z-chunk, register bound and grid of the batched launch
+        (resets every shot); returns ((ty, ny, pty, pny, zchunk, waves),
+        grid, ms per step)."""
+        t, g, ms = _Tile(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().sg_batch_autotune(self._h, int(steps), ctypes.byref(t), ctypes.byref(g), ctypes.byref(ms)))
+        return t.as_tuple(), g.value, ms.value
 
     def step(self, n=1):
         _check(lib().sg_batch_step(self._h, int(n)))

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol(sgfd_mod):
 
 
 def test_api_version(sgfd_mod):
-    assert sgfd_mod.lib().sg_api_version() == 1
+    assert sgfd_mod.lib().sg_api_version() == 2
 
 
 def test_ricker_bitexact_with_oracle(sgfd_mod, oracle_mod):
@@ -143,4 +143,4 @@ def test_c_header_is_plain_c99(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I" + os.path.join(ROOT, "include"),
                     str(src), "-L" + lib_dir, "-lsgfd", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=60).stdout.split()
-    assert out[0] == "1" and out[1] == "0" and abs(float(out[2]) - 19845 / 16384) < 1e-6
+    assert out[0] == "2" and out[1] == "0" and abs(float(out[2]) - 19845 / 16384) < 1e-6

@@ -87,9 +87,10 @@ def test_autotune_keeps_results(sgfd_mod):
     tr0, F0 = _run(wl)
     s = wl.solver()
     try:
-        (ty, ny, pty, pny, zc), ms = s.autotune(2)
+        (ty, ny, pty, pny, zc, wv), ms = s.autotune(2)
         assert ty in (4, 8, 16) and pty in (4, 8, 16) and ny in (1, 2) and pny in (1, 2) and ms > 0
-        assert s.tile() == (ty, ny, pty, pny, zc)
+        assert wv in (0, 2, 3, 4)
+        assert s.tile() == (ty, ny, pty, pny, zc, wv)
         assert s.steps_done == 0
         s.step(wl.nt)
         s.synchronize()
@@ -100,7 +101,11 @@ def test_autotune_keeps_results(sgfd_mod):
 
 
 @pytest.mark.parametrize("launch", [dict(ty=16, ny=1, pty=8, pny=1, zchunk=32, grid=0, graph=True),
-                                    dict(ty=16, ny=2, pty=4, pny=1, zchunk=13, grid=64, graph=False)])
+                                    dict(ty=16, ny=2, pty=4, pny=1, zchunk=13, grid=64, graph=False),
+                                    dict(ty=4, pty=4, zchunk=24, grid=0, graph=True, waves=0),
+                                    dict(ty=4, pty=4, zchunk=16, grid=96, graph=True, waves=2),
+                                    dict(ty=4, pty=4, zchunk=32, grid=0, graph=False, waves=3),
+                                    dict(ty=4, pty=4, zchunk=9, grid=40, graph=True, waves=4)])
 def test_batched_shots_match_single_shots(sgfd_mod, launch):
     sgfd = sgfd_mod
     dims = [(96, 80, 60), (130, 72, 64), (70, 101, 50), (64, 64, 80)]
@@ -295,3 +300,32 @@ def test_failed_station_request_leaves_the_solver_unchanged(sgfd_mod):
         s.step(wl.nt)
         assert np.array_equal(s.stations(), ref[0])
         assert np.array_equal(s.fields(), ref[1])
+
+
+def test_batch_autotune_keeps_results(sgfd_mod):
+    """The batch autotuner (z-chunk, register bound, grid of the one-launch
+    form, two per-list shapes) resets every shot; stepping afterwards gives
+    the single shots' results bit for bit."""
+    sgfd = sgfd_mod
+    dims = [(96, 80, 60), (70, 101, 50), (64, 64, 80)]
+    wls = [_small_workload(sgfd, 20 + i, *d) for i, d in enumerate(dims)]
+    _say(f"batch autotune, {len(wls)} shots")
+    ref = [_run(wl) for wl in wls]
+    solvers = [wl.solver() for wl in wls]
+    b = None
+    try:
+        b = sgfd.Batch(solvers)
+        tile, grid, ms = b.autotune(2)
+        _say(f"chose {tile} grid {grid}, {ms:.3f} ms per step")
+        assert ms > 0 and tile[5] in (0, 2, 3, 4)
+        assert all(s.steps_done == 0 for s in solvers)
+        b.step(wls[0].nt)
+        b.synchronize()
+        for s, (tr0, F0) in zip(solvers, ref):
+            assert np.array_equal(s.fields(), F0)
+            assert np.array_equal(s.stations(), tr0)
+    finally:
+        if b is not None:
+            b.close()
+        for s in solvers:
+            s.close()
