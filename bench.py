@@ -228,7 +228,12 @@ def precision_study(args):
     workload (the same source, medium and stations); its Gpt-updates/s
     (HIP events over the timed steps) and its seismogram rel-L2 against the
     FP64 build's traces.  16-bit storage uses scales from an FP32 pilot run.
-    Reports the FP16/FP32 speed ratio and whether FP16 meets rel-L2 <= 1e-3."""
+    Reports the FP16/FP32 speed ratio and whether FP16 meets rel-L2 <= 1e-3.
+    Two diagnostic rows split the FP16 error: FP32 storage with only the
+    model (b, lambda, mu) rounded to FP16 under its power-of-two scales (the
+    model's share), and FP16 storage with every field scale multiplied by
+    1/8 and by 4 (if the error does not move, the rounding of the 11-bit
+    significand sets it, not the scale, so finer scales cannot help)."""
     import numpy as np
     from sgfd import configs
 
@@ -236,15 +241,24 @@ def precision_study(args):
     K = wl.nt
     res = {}
     ref = None
-    # (name, storage, rounding, compensated fields)
-    modes = [("fp64", "fp64", None, None), ("fp32", "fp32", None, None), ("fp16", "fp16", None, None),
-             ("bf16", "bf16", None, None), ("fp16-stochastic", "fp16", "stochastic", None),
-             ("fp16-compensated", "fp16", None, "all"), ("bf16-stochastic", "bf16", "stochastic", None),
-             ("bf16-compensated", "bf16", None, "all")]
-    for name, prec, rounding, comp in modes:
+    # (name, storage, rounding, compensated fields, variant)
+    modes = [("fp64", "fp64", None, None, None), ("fp32", "fp32", None, None, None),
+             ("fp16", "fp16", None, None, None), ("bf16", "bf16", None, None, None),
+             ("fp16-stochastic", "fp16", "stochastic", None, None), ("fp16-compensated", "fp16", None, "all", None),
+             ("bf16-stochastic", "bf16", "stochastic", None, None), ("bf16-compensated", "bf16", None, "all", None),
+             ("fp32-model-fp16", "fp32", None, None, "model16"), ("fp16-scales-x1/8", "fp16", None, None, 0.125),
+             ("fp16-scales-x4", "fp16", None, None, 4.0)]
+    if args.quick_study:
+        modes = [m for m in modes if m[0] in ("fp64", "fp32", "fp16", "fp32-model-fp16", "fp16-scales-x1/8",
+                                              "fp16-scales-x4")]
+    for name, prec, rounding, comp, variant in modes:
         log(f"bench: {name}, {K} steps")
         s = wl.solver(precision=prec)
         try:
+            if variant == "model16":
+                s.set_medium(*[model_rounded_fp16(a) for a in (wl.b, wl.lam, wl.mu)])
+            elif variant is not None:
+                s.set_field_scales(wl._scales * np.float32(variant))
             if rounding:
                 s.set_rounding(rounding)
             if comp:
@@ -269,6 +283,18 @@ def precision_study(args):
     print(json.dumps(out))
 
 
+def model_rounded_fp16(a):
+    """The model array as FP16 storage holds it: scaled by the power of two
+    that maps its largest |value| into [2^13, 2^14), rounded to half, scaled
+    back (float32)."""
+    import numpy as np
+
+    a = np.asarray(a, np.float32)
+    m = float(np.abs(a).max())
+    sc = np.float32(2.0 ** np.floor(np.log2(16384.0 / m)))
+    return ((a * sc).astype(np.float16).astype(np.float32) / sc).astype(np.float32)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,6 +313,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
     ap.add_argument("--compare-sequential", action="store_true",
                     help="batched32: also time the shots one solver at a time")
+    ap.add_argument("--quick-study", action="store_true",
+                    help="precision study: only FP64, FP32, FP16 and the FP16 error-split rows")
     ap.add_argument("--precision-study", action="store_true",
                     help="every storage precision and 16-bit mode over the full workload: speed and seismogram "
                          "error against FP64")
