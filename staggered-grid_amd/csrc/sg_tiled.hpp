@@ -87,6 +87,71 @@ __device__ __forceinline__ C lbwd(const C *f, int c, int st, const C *k)
     return r;
 }
 
+// Cross-lane x taps (SG_XLANE=1, FP32 arithmetic).  Lane l of a wave holds
+// column x0+l of one row, and h the row's x halo: lanes 0-4 columns
+// x0+64..x0+68, lanes 59-63 columns x0-5..x0-1.  The neighbours at x0+l+n
+// come from n wave-wide shifts by one lane (DPP wave_shl:1); the lane that
+// falls off the end (63) takes the next right-halo value, which a rotation
+// of h by one lane (wave_rol:1) brings to lane 63 at each step.  Likewise
+// to the left with wave_shr:1 / wave_ror:1 and lane 0.  The sums are the
+// ones of lfwd/lbwd, term for term, so the results are the same bits.
+#ifndef SG_XLANE
+#define SG_XLANE 0
+#endif
+template <int CTRL>
+__device__ __forceinline__ float dpp(float old, float src)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xf, 0xf, false));
+}
+constexpr int DPP_SHL1 = 0x130, DPP_ROL1 = 0x134, DPP_SHR1 = 0x138, DPP_ROR1 = 0x13C;
+
+// p[n-1] = f[x+n] for n <= NP, m[n-1] = f[x-n] for n <= NM
+template <int NP, int NM>
+__device__ __forceinline__ void xnb(float c, float h, float (&p)[5], float (&m)[5])
+{
+    float t = h, v = c;
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+        t = dpp<DPP_ROL1>(t, t);
+        v = dpp<DPP_SHL1>(t, v);
+        p[n] = v;
+    }
+    float u = h, w = c;
+#pragma unroll
+    for (int n = 0; n < NM; ++n) {
+        u = dpp<DPP_ROR1>(u, u);
+        w = dpp<DPP_SHR1>(u, w);
+        m[n] = w;
+    }
+}
+
+// lfwd / lbwd along x with stride 1, from a lane's own value and the halo
+__device__ __forceinline__ float xfwd(float c, float h, const float *k)
+{
+    float p[5], m[5];
+    xnb<5, 4>(c, h, p, m);
+    float r = k[0] * (p[0] - c);
+    r = fmad(k[1], p[1] - m[0], r);
+    r = fmad(k[2], p[2] - m[1], r);
+    r = fmad(k[3], p[3] - m[2], r);
+    r = fmad(k[4], p[4] - m[3], r);
+    return r;
+}
+__device__ __forceinline__ float xbwd(float c, float h, const float *k)
+{
+    float p[5], m[5];
+    xnb<4, 5>(c, h, p, m);
+    float r = k[0] * (c - m[0]);
+    r = fmad(k[1], p[0] - m[1], r);
+    r = fmad(k[2], p[1] - m[2], r);
+    r = fmad(k[3], p[2] - m[3], r);
+    r = fmad(k[4], p[3] - m[4], r);
+    return r;
+}
+// the double overloads are never called (FP64 keeps the LDS taps)
+__device__ __forceinline__ double xfwd(double, double, const double *) { return 0.0; }
+__device__ __forceinline__ double xbwd(double, double, const double *) { return 0.0; }
+
 // Global-address-space access.  The shot descriptor lives in device memory,
 // so the compiler cannot infer that its pointers are global and would emit
 // FLAT instructions, which also count against the LDS wait counter and make
@@ -262,8 +327,8 @@ struct TileGeo {
     unsigned ro[NY];    // clamped offset of (i, jr) in a plane
     bool rin[NY];       // (i, jr) inside the array
     bool rup[NY];       // (i, jr) updated by this box
-    unsigned hro[NY];   // x-halo offsets (threads tx < 10)
-    bool hok[NY];
+    unsigned hro[NY];   // x-halo offsets (threads tx < 10; with XL lanes
+    bool hok[NY];       // 0-4 right of the tile and 59-63 left of it)
     int hc;             // LDS column of this thread's x-halo value
     unsigned yro[NH];   // y-halo offsets: rows h = ty + m*BY (h < 10)
     bool yok[NH];
@@ -274,7 +339,7 @@ struct TileGeo {
     bool byok, bxyok;
 };
 
-template <int TY, int NY, int ES>
+template <int TY, int NY, int ES, bool XL = false>
 __device__ __forceinline__ void tile_geo(const Geom &g, int i0, int i1, int j0, int j1, TileGeo<TY, NY> &t)
 {
     constexpr int BY = TY / NY;  // ES: bytes per stored element
@@ -284,8 +349,9 @@ __device__ __forceinline__ void tile_geo(const Geom &g, int i0, int i1, int j0, 
     const bool xin = t.i < g.nx, xup = t.i >= HALO && t.i < i1;
     t.ic = xin ? t.i : g.nx - 1;
     t.hc = tx < 5 ? tx : TX + tx;
-    const int hi = i0 - HALO + t.hc;  // negative left of the first tile
-    const bool hx = tx < 10 && hi >= 0 && hi < g.nx;
+    // negative left of the first tile
+    const int hi = XL ? (tx < 5 ? i0 + TX + tx : i0 + tx - TX) : i0 - HALO + t.hc;
+    const bool hx = (XL ? (tx < 5 || tx >= TX - 5) : tx < 10) && hi >= 0 && hi < g.nx;
     const int hic = hi < 0 ? 0 : (hi < g.nx ? hi : g.nx - 1);
     const int bxi = i0 + TX;
     const int bxc = bxi < g.nx ? bxi : g.nx - 1;
@@ -345,8 +411,10 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     const Geom &g = s.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const long long sz = g.sz;
+    // x taps across lanes (FP32 arithmetic only; 1 = interior boxes, 2 = all)
+    constexpr bool XL = (SG_XLANE == 2 || (SG_XLANE == 1 && DM == 0)) && sizeof(C) == 4;
     TileGeo<TY, NY> t;
-    tile_geo<TY, NY, (int)sizeof(T)>(g, i0, i1, j0, j1, t);
+    tile_geo<TY, NY, (int)sizeof(T), XL>(g, i0, i1, j0, j1, t);
 
     const T *Fxx = s.f[SXX], *Fyy = s.f[SYY], *Fzz = s.f[SZZ];
     const T *Fxy = s.f[SXY], *Fxz = s.f[SXZ], *Fyz = s.f[SYZ];
@@ -446,18 +514,28 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     }
 
     for (int k = k0; k < k1; ++k) {
+        // x derivatives across lanes, before the staging registers are reused
+        C dxl[3][NY];
+        if (XL) {
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                dxl[0][r] = xfwd(c_xx[r], h_xx[r], s.cx);
+                dxl[1][r] = xbwd(c_xy[r], h_xy[r], s.cx);
+                dxl[2][r] = xbwd(qxz[r][5], h_xz[r], s.cx);
+            }
+        }
         // ---- stage plane k
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
-            L.sxx[lr][tx + 5] = c_xx[r];
+            if (!XL) L.sxx[lr][tx + 5] = c_xx[r];
             L.sxy[lr + 5][tx + 5] = c_xy[r];
             L.syy[lr + 5][tx] = c_yy[r];
-            L.sxz[lr][tx + 5] = qxz[r][5];
+            if (!XL) L.sxz[lr][tx + 5] = qxz[r][5];
             L.syz[lr + 5][tx] = qyz[r][5];
             L.b[lr][tx] = qb[r][0];
         }
-        if (tx < 10) {
+        if (!XL && tx < 10) {
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 const int lr = ty * NY + r;
@@ -555,13 +633,13 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
             C d[3][3];
-            d[0][0] = lfwd<C>(&L.sxx[lr][0], tx + 5, 1, s.cx);
+            d[0][0] = XL ? dxl[0][r] : lfwd<C>(&L.sxx[lr][0], tx + 5, 1, s.cx);
             d[0][1] = lbwd<C>(&L.sxy[0][tx + 5], lr + 5, XW, s.cy);
             d[0][2] = lbwd<C>(qxz[r], 5, 1, s.cz);
-            d[1][0] = lbwd<C>(&L.sxy[lr + 5][0], tx + 5, 1, s.cx);
+            d[1][0] = XL ? dxl[1][r] : lbwd<C>(&L.sxy[lr + 5][0], tx + 5, 1, s.cx);
             d[1][1] = lfwd<C>(&L.syy[0][tx], lr + 5, TX, s.cy);
             d[1][2] = lbwd<C>(qyz[r], 5, 1, s.cz);
-            d[2][0] = lbwd<C>(&L.sxz[lr][0], tx + 5, 1, s.cx);
+            d[2][0] = XL ? dxl[2][r] : lbwd<C>(&L.sxz[lr][0], tx + 5, 1, s.cx);
             d[2][1] = lbwd<C>(&L.syz[0][tx], lr + 5, TX, s.cy);
             d[2][2] = lfwd<C>(qzz[r], 4, 1, s.cz);
             const C b0 = L.b[lr][tx];
@@ -649,8 +727,10 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     const Geom &g = s.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const long long sz = g.sz;
+    // x taps across lanes (FP32 arithmetic only; 1 = interior boxes, 2 = all)
+    constexpr bool XL = (SG_XLANE == 2 || (SG_XLANE == 1 && DM == 0)) && sizeof(C) == 4;
     TileGeo<TY, NY> t;
-    tile_geo<TY, NY, (int)sizeof(T)>(g, i0, i1, j0, j1, t);
+    tile_geo<TY, NY, (int)sizeof(T), XL>(g, i0, i1, j0, j1, t);
 
     const T *Fvx = s.f[VX], *Fvy = s.f[VY], *Fvz = s.f[VZ];
     const T *Fmu = s.mu, *Flam = s.lam;
@@ -776,6 +856,16 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     const unsigned src_o = s.src >= 0 ? (unsigned)(s.src - (long long)src_k * sz) * (unsigned)sizeof(T) : 0u;
 
     for (int k = k0; k < k1; ++k) {
+        // x derivatives across lanes, before the staging registers are reused
+        C dxl[3][NY];
+        if (XL) {
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                dxl[0][r] = xbwd(qvx[r][4], h_vx[r], s.cx);
+                dxl[1][r] = xfwd(qvy[r][4], h_vy[r], s.cx);
+                dxl[2][r] = xfwd(qvz[r][5], h_vz[r], s.cx);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
@@ -783,7 +873,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
             L.vy[lr + 5][tx + 5] = qvy[r][4];
             L.vz[lr + 5][tx + 5] = qvz[r][5];
         }
-        if (tx < 10) {
+        if (!XL && tx < 10) {
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 const int lr = ty * NY + r;
@@ -903,12 +993,12 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
-            const C dX = lbwd<C>(&L.vx[lr + 5][0], tx + 5, 1, s.cx);
+            const C dX = XL ? dxl[0][r] : lbwd<C>(&L.vx[lr + 5][0], tx + 5, 1, s.cx);
             const C dY = lbwd<C>(&L.vy[0][tx + 5], lr + 5, XW, s.cy);
             const C dZ = lbwd<C>(qvz[r], 5, 1, s.cz);
-            const C dxy_x = lfwd<C>(&L.vy[lr + 5][0], tx + 5, 1, s.cx);
+            const C dxy_x = XL ? dxl[1][r] : lfwd<C>(&L.vy[lr + 5][0], tx + 5, 1, s.cx);
             const C dxy_y = lfwd<C>(&L.vx[0][tx + 5], lr + 5, XW, s.cy);
-            const C dxz_x = lfwd<C>(&L.vz[lr + 5][0], tx + 5, 1, s.cx);
+            const C dxz_x = XL ? dxl[2][r] : lfwd<C>(&L.vz[lr + 5][0], tx + 5, 1, s.cx);
             const C dxz_z = lfwd<C>(qvx[r], 4, 1, s.cz);
             const C dyz_y = lfwd<C>(&L.vz[0][tx + 5], lr + 5, XW, s.cy);
             const C dyz_z = lfwd<C>(qvy[r], 4, 1, s.cz);
