@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage leg of an FP32 run")
     ap.add_argument("--no-tune", action="store_true", help="batched32: keep the default launch (no autotuner)")
+    ap.add_argument("--tune", action="store_true",
+                    help="single-shot workloads: run the solver autotuner (sg_autotune) before timing")
     ap.add_argument("--streams", type=int, default=None, help="streams per sweep (1, 2, 3)")
     ap.add_argument("--copy-calib", action="store_true", help="only measure the copy-kernel bandwidth")
     ap.add_argument("--compare-sequential", action="store_true",
@@ -376,6 +378,7 @@ def main():
                    "precision": args.precision, "kernel": args.kernel, "parallelism": "single-gpu"},
         "roofline": roof,
         "sweeps_ms": r["sweeps_ms"],
+        "launch": r["launch"],
         "step_hbm_frac": r["step_hbm_frac"],
         "energy_after": r["energy"],
     }
@@ -410,6 +413,12 @@ def timed_run(sgfd, wl, args, precision):
             s.set_tile(*args.tile)
         if args.streams:
             s.set_streams(args.streams)
+        launch = None
+        if args.tune and args.kernel == "tiled":
+            log(f"bench: {precision} autotune")
+            tile, tms = s.autotune(4)
+            launch = {"tile": list(tile), "streams": s.streams(), "autotune_ms_per_step": tms}
+            log(f"bench: autotune chose tile {tile}, {launch['streams']} stream(s) ({tms:.3f} ms per step)")
         log(f"bench: {precision} warmup {args.warmup} steps")
         s.step(args.warmup)
         s.synchronize()
@@ -427,7 +436,10 @@ def timed_run(sgfd, wl, args, precision):
     roof = {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": read_traffic(args.workload, precision),
             "alg_bytes_per_launch": dom[2], "avg_launch_ms": dom[1]}
-    return {"ms_total": ms_tot, "roofline": roof, "sweeps_ms": {"velocity_avg": avg_v, "stress_avg": avg_s},
+    if launch is not None:
+        roof["traffic"] = None  # the committed traffic summary covers the default launch only
+    return {"ms_total": ms_tot, "roofline": roof, "launch": launch,
+            "sweeps_ms": {"velocity_avg": avg_v, "stress_avg": avg_s},
             "step_hbm_frac": (vbytes + sbytes) * K / (ms_tot / 1e3) / 1e9 / HBM_PEAK_GBPS, "energy": list(energy)}
 
 

@@ -292,6 +292,7 @@ int sg_get_tile(const sg_solver *s, sg_tile *tile);
  * or 1 for grids below 4 Mi points (launch-bound).  The lists write disjoint
  * points, so results do not change. */
 int sg_set_streams(sg_solver *s, int nstreams);
+int sg_get_streams(const sg_solver *s, int *nstreams);
 /* Autotuner: times `steps` steps for a set of interior shapes, PML shapes
  * and z-chunk lengths, then (FP32 storage) the one-launch-per-sweep form on
  * one stream with each register bound (waves_per_simd 0, 2, 3, 4); keeps the

@@ -1171,6 +1171,14 @@ int sg_set_streams(sg_solver *s, int nstreams)
     return SG_OK;
 }
 
+int sg_get_streams(const sg_solver *s, int *nstreams)
+{
+    if (int rc = check_solver(s)) return rc;
+    if (!nstreams) return fail(SG_ERR_INVALID, "sg_get_streams: null output");
+    *nstreams = s->nstreams;
+    return SG_OK;
+}
+
 int sg_set_rounding(sg_solver *s, int mode)
 {
     if (int rc = check_solver(s)) return rc;
@@ -1286,7 +1294,7 @@ int sg_autotune(sg_solver *s, int steps, sg_tile *best_tile, double *best_ms)
     // The state is reset afterwards.
     const int uz = s->g.uz;
     const int shapes[][2] = {{4, 1}, {8, 1}, {16, 1}, {8, 2}, {16, 2}};
-    const int zcs[] = {0, 8, 12, 16, 24, 48};
+    const int zcs[] = {0, 8, 10, 12, 16, 20, 24, 32, 48};
     double best = 1e300;
     Tile bt = s->tile;
     const int ns0 = s->nstreams;

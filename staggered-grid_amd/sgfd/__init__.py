@@ -111,6 +111,7 @@ API = {
     "sg_set_rounding": (ctypes.c_int, [_VP, ctypes.c_int]),
     "sg_set_damping_local": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "sg_set_streams": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "sg_get_streams": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
     "sg_set_compensation": (ctypes.c_int, [_VP, ctypes.c_uint]),
     "sg_step_timed": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP]),
     "sg_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -466,6 +467,11 @@ class Solver:
     def set_streams(self, n):
         """Streams the work-lists of a sweep are spread over (1, 2 or 3)."""
         _check(lib().sg_set_streams(self._h, int(n)))
+
+    def streams(self):
+        n = ctypes.c_int()
+        _check(lib().sg_get_streams(self._h, ctypes.byref(n)))
+        return n.value
 
     def set_rounding(self, mode):
         """16-bit storage rounding: "nearest" (even) or "stochastic"."""

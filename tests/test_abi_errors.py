@@ -71,6 +71,7 @@ def test_create_rejects_null_arguments(L):
     ("sg_reset", ()),
     ("sg_set_graph", (1,)),
     ("sg_set_streams", (2,)),
+    ("sg_get_streams", (ctypes.byref(ctypes.c_int()),)),
     ("sg_set_rounding", (0,)),
     ("sg_set_compensation", (0,)),
     ("sg_set_source_sample", (0, 1.0)),

@@ -70,6 +70,7 @@ def test_stream_counts_are_bit_identical(sgfd_mod):
             s = wl.solver()
             try:
                 s.set_streams(n)
+                assert s.streams() == n
                 s.set_graph(graph)
                 s.step(wl.nt)
                 s.synchronize()
@@ -91,6 +92,7 @@ def test_autotune_keeps_results(sgfd_mod):
         assert ty in (4, 8, 16) and pty in (4, 8, 16) and ny in (1, 2) and pny in (1, 2) and ms > 0
         assert wv in (0, 2, 3, 4)
         assert s.tile() == (ty, ny, pty, pny, zc, wv)
+        assert s.streams() in (1, 2, 3) and (wv == 0 or s.streams() == 1)
         assert s.steps_done == 0
         s.step(wl.nt)
         s.synchronize()
