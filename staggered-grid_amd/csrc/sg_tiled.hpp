@@ -63,6 +63,49 @@ struct StrLds {
     C rm[2][TY + 1][TX + 1];  // 1/mu of planes k and k+1 (ring)
 };
 
+// Double-buffered staging (SG_DB=1): plane k is staged into buffer k & 1 and
+// the 1/mu ring has three slots, so a wave may stage plane k+1 while others
+// still read plane k, and each plane needs one barrier instead of two.  Off
+// by default: the same bits, 0.7-1.2 % slower at 512x512x384 and 500^3 on
+// MI355X (twice the LDS per block; the waves do not wait at the barrier).
+#ifndef SG_DB
+#define SG_DB 0
+#endif
+template <class C, int TY>
+struct VelLdsDB {
+    VelLds<C, TY> b[2];
+};
+template <class C, int TY>
+struct StrV {
+    static constexpr int YH = TY + 10;
+    C vx[YH][XW];
+    C vy[YH][XW];
+    C vz[YH][XW];
+};
+template <class C, int TY>
+struct StrLdsDB {
+    StrV<C, TY> v[2];
+    C rm[3][TY + 1][TX + 1];
+};
+template <class C, int TY>
+__device__ __forceinline__ VelLds<C, TY> &vplane(VelLds<C, TY> &L, int) { return L; }
+template <class C, int TY>
+__device__ __forceinline__ VelLds<C, TY> &vplane(VelLdsDB<C, TY> &L, int k) { return L.b[k & 1]; }
+template <class C, int TY>
+__device__ __forceinline__ StrLds<C, TY> &vplane(StrLds<C, TY> &L, int) { return L; }
+template <class C, int TY>
+__device__ __forceinline__ StrV<C, TY> &vplane(StrLdsDB<C, TY> &L, int k) { return L.v[k & 1]; }
+template <class C, int TY>
+__device__ __forceinline__ C (&rmslot(StrLds<C, TY> &L, int k))[TY + 1][TX + 1] { return L.rm[k & 1]; }
+template <class C, int TY>
+__device__ __forceinline__ C (&rmslot(StrLdsDB<C, TY> &L, int k))[TY + 1][TX + 1] { return L.rm[k % 3]; }
+template <class LS>
+struct IsDB : std::false_type {};
+template <class C, int TY>
+struct IsDB<VelLdsDB<C, TY>> : std::true_type {};
+template <class C, int TY>
+struct IsDB<StrLdsDB<C, TY>> : std::true_type {};
+
 // 5-pair staggered sums: forward at c uses f[c+n] - f[c+1-n], backward uses
 // f[c+n-1] - f[c-n], n = 1..5, with stride st.
 template <class C>
@@ -403,9 +446,9 @@ __device__ __forceinline__ void store_comp(const Shot<T, C> &s, int f, int k, un
 // ---------------------------------------------------------------------------
 // velocity sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int TY, int NY, int DM, bool CP>
+template <class T, class C, int TY, int NY, int DM, bool CP, class LS>
 __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
-                                              VelLds<C, TY> &L)
+                                              LS &Ls)
 {
     constexpr int BY = TY / NY, NH = TileGeo<TY, NY>::NH;
     const Geom &g = s.g;
@@ -514,6 +557,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     }
 
     for (int k = k0; k < k1; ++k) {
+        auto &L = vplane(Ls, k);
         // x derivatives across lanes, before the staging registers are reused
         C dxl[3][NY];
         if (XL) {
@@ -696,7 +740,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 }
             }
         }
-        __syncthreads();
+        if constexpr (!IsDB<LS>::value) __syncthreads();
 
         // ---- advance the z queues
 #pragma unroll
@@ -719,9 +763,9 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 // ---------------------------------------------------------------------------
 // stress sweep
 // ---------------------------------------------------------------------------
-template <class T, class C, int TY, int NY, int DM, bool CP>
+template <class T, class C, int TY, int NY, int DM, bool CP, class LS>
 __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1, int j0, int j1, int k0, int k1,
-                                              StrLds<C, TY> &L)
+                                              LS &Ls)
 {
     constexpr int BY = TY / NY, NH = TileGeo<TY, NY>::NH;
     const Geom &g = s.g;
@@ -766,16 +810,16 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     // reciprocals into the ring slot of plane k (0 outside the array: never
     // read by an updated point)
     auto stage_mu = [&](int k) {
-        const int sl = k & 1;
+        auto &R = rmslot(Ls, k);
 #pragma unroll
-        for (int r = 0; r < NY; ++r) L.rm[sl][ty * NY + r][tx] = t.rin[r] ? (C)1 / m_c[r] : (C)0;
+        for (int r = 0; r < NY; ++r) R[ty * NY + r][tx] = t.rin[r] ? (C)1 / m_c[r] : (C)0;
         if (tx == 10) {
 #pragma unroll
-            for (int r = 0; r < NY; ++r) L.rm[sl][ty * NY + r][TX] = (t.bxok && t.rin[r]) ? (C)1 / m_x[r] : (C)0;
+            for (int r = 0; r < NY; ++r) R[ty * NY + r][TX] = (t.bxok && t.rin[r]) ? (C)1 / m_x[r] : (C)0;
         }
         if (ty == 0) {
-            L.rm[sl][TY][tx] = t.byok ? (C)1 / m_y : (C)0;
-            if (tx == 10) L.rm[sl][TY][TX] = t.bxyok ? (C)1 / m_xy : (C)0;
+            R[TY][tx] = t.byok ? (C)1 / m_y : (C)0;
+            if (tx == 10) R[TY][TX] = t.bxyok ? (C)1 / m_xy : (C)0;
         }
     };
     C muk[NY], mun[NY];  // mu of planes k and k+1 (for lambda + 2 mu)
@@ -856,6 +900,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     const unsigned src_o = s.src >= 0 ? (unsigned)(s.src - (long long)src_k * sz) * (unsigned)sizeof(T) : 0u;
 
     for (int k = k0; k < k1; ++k) {
+        auto &L = vplane(Ls, k);
         // x derivatives across lanes, before the staging registers are reused
         C dxl[3][NY];
         if (XL) {
@@ -989,7 +1034,8 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
             }
         }
 
-        const int s0 = k & 1, s1 = s0 ^ 1;
+        auto &R0 = rmslot(Ls, k);
+        auto &R1 = rmslot(Ls, k + 1);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
@@ -1002,9 +1048,9 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
             const C dxz_z = lfwd<C>(qvx[r], 4, 1, s.cz);
             const C dyz_y = lfwd<C>(&L.vz[0][tx + 5], lr + 5, XW, s.cy);
             const C dyz_z = lfwd<C>(qvy[r], 4, 1, s.cz);
-            const C r0 = L.rm[s0][lr][tx], rx = L.rm[s0][lr][tx + 1], ry = L.rm[s0][lr + 1][tx];
-            const C rxy = L.rm[s0][lr + 1][tx + 1];
-            const C rz = L.rm[s1][lr][tx], rxz = L.rm[s1][lr][tx + 1], ryz = L.rm[s1][lr + 1][tx];
+            const C r0 = R0[lr][tx], rx = R0[lr][tx + 1], ry = R0[lr + 1][tx];
+            const C rxy = R0[lr + 1][tx + 1];
+            const C rz = R1[lr][tx], rxz = R1[lr][tx + 1], ryz = R1[lr + 1][tx];
             const C muxy = (C)4 / (((r0 + rx) + ry) + rxy);
             const C muxz = (C)4 / (((r0 + rx) + rz) + rxz);
             const C muyz = (C)4 / (((r0 + ry) + rz) + ryz);
@@ -1094,7 +1140,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 }
             }
         }
-        __syncthreads();
+        if constexpr (!IsDB<LS>::value) __syncthreads();
 
         // ---- ring and queues advance
         if (more) {
@@ -1159,12 +1205,13 @@ struct LeadOn {
 // the LDS of a velocity / stress box for the chosen form
 template <class T, class C, int TY, int NY, int DM>
 struct VelLdsOf {
-    using type = typename std::conditional<LeadOn<T, DM, 0, TY, NY>::v, VelLdsL<C, TY>, VelLds<C, TY>>::type;
+    using type = typename std::conditional<LeadOn<T, DM, 0, TY, NY>::v, VelLdsL<C, TY>,
+                                           typename std::conditional<SG_DB != 0, VelLdsDB<C, TY>, VelLds<C, TY>>::type>::type;
 };
 template <class T, class C, int TY, int NY, int DM>
 struct StrLdsOf {
     using type = typename std::conditional<LeadOn<T, DM, 1, TY, NY>::v, StrLdsL<C, TY, DM == 0 ? 5 : 6>,
-                                           StrLds<C, TY>>::type;
+                                           typename std::conditional<SG_DB != 0, StrLdsDB<C, TY>, StrLds<C, TY>>::type>::type;
 };
 
 template <class T, class C, int TY, int NY, int DM, bool CP, class LT>
@@ -1331,8 +1378,11 @@ struct FusedLds {
     static constexpr int mask = sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16)
                                                : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
     using type = typename std::conditional<
-        W == 0, typename std::conditional<(mask & 0x1F) != 0, VelLdsL<C, 4>, VelLds<C, 4>>::type,
-        typename std::conditional<(mask & 0x1F) != 0, StrLdsL<C, 4, 6>, StrLds<C, 4>>::type>::type;
+        W == 0,
+        typename std::conditional<(mask & 0x1F) != 0, VelLdsL<C, 4>,
+                                  typename std::conditional<SG_DB != 0, VelLdsDB<C, 4>, VelLds<C, 4>>::type>::type,
+        typename std::conditional<(mask & 0x1F) != 0, StrLdsL<C, 4, 6>,
+                                  typename std::conditional<SG_DB != 0, StrLdsDB<C, 4>, StrLds<C, 4>>::type>::type>::type;
 };
 
 // WV: minimum waves per SIMD the register allocation must allow (0: no
