@@ -270,7 +270,9 @@ int sg_set_graph(sg_solver *s, int enable);
  * and z_chunk planes deep; a block of 64 x tile_y/rows_per_thread threads
  * computes it.  Interior boxes (no damped point) and PML boxes have their
  * own shape.  (tile_y, rows_per_thread) is one of (4,1) (8,1) (8,2) (16,1)
- * (16,2); z_chunk 0 = automatic.  waves_per_simd bounds the registers of
+ * (16,2); z_chunk 0 = automatic (the PML thickness P when 8 <= P <= 32,
+ * so every box marches as many planes as the whole z PML slabs; else 24,
+ * fewer on grids with few columns).  waves_per_simd bounds the registers of
  * the one-launch-per-sweep form (the five work-lists in one kernel, used by
  * a one-stream solver with (4,1) boxes throughout and by batches with (4,1)
  * boxes), so that at least that many waves fit on a SIMD: 0 = no bound, or
