@@ -306,6 +306,38 @@ __device__ __forceinline__ C mld(const T *__restrict__ p, unsigned boff, bool ok
     return ok ? v : (C)0;
 }
 
+// The same in two halves: a load of the stored value, and its conversion and
+// mask where the value is first used.  Any arithmetic on a loaded register
+// makes the compiler wait for that load right there; with the conversion and
+// the mask at the use (the next plane's staging, the update, the queue push)
+// the loads issued for plane k+1 stay in flight while plane k is computed.
+template <class T, class C>
+__device__ __forceinline__ C cvm(T raw, bool ok, float inv)
+{
+    const C v = Cvt<T, C>::ld(raw, inv);
+    return ok ? v : (C)0;
+}
+
+// Staging registers of the sweeps: the stored value, converted and masked at
+// its use (FP32, FP16, BF16), or for FP64, the checking build, the converted
+// and masked value itself, as loaded (its loads are waited for where they are
+// issued).
+template <class T, class C>
+struct Stg {
+    static constexpr bool late = sizeof(T) != 8;
+    using S = typename std::conditional<late, T, C>::type;
+    static __device__ __forceinline__ S ld(const T *__restrict__ p, unsigned boff, bool ok, float inv)
+    {
+        if constexpr (late) return bld(p, boff);
+        else return mld<T, C>(p, boff, ok, inv);
+    }
+    static __device__ __forceinline__ C use(S v, bool ok, float inv)
+    {
+        if constexpr (late) return cvm<T, C>(v, ok, inv);
+        else return v;
+    }
+};
+
 // Per-cell damping (the CUDA reference's form): A holds dt*g/2 per axis on
 // entry; with the point's local S velocity v, hd = A*v and
 // A <- (1-hd)/(1+hd), B <- 1/(1+hd).
@@ -485,56 +517,65 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         qb[r][1] = mld<T, C>(plane(Fb, k0 + 1, sz), t.ro[r], t.rin[r], ib);
     }
 
-    // staging registers of one plane
-    C c_xx[NY], c_xy[NY], c_yy[NY], h_xx[NY], h_xy[NY], h_xz[NY], b_x[NY];
-    C y_yy[NH], y_yz[NH], y_xy[NH], b_y;
+    // staging registers of one plane: stored values, converted and masked
+    // when they are staged
+    using G = Stg<T, C>;
+    using S = typename G::S;
+    S c_xx[NY], c_xy[NY], c_yy[NY], h_xx[NY], h_xy[NY], h_xz[NY], b_x[NY];
+    S y_yy[NH], y_yz[NH], y_xy[NH], b_y;
     auto fetch = [&](int k) {
         const T *pxx = plane(Fxx, k, sz), *pxy = plane(Fxy, k, sz), *pyy = plane(Fyy, k, sz);
         const T *pxz = plane(Fxz, k, sz), *pyz = plane(Fyz, k, sz), *pb = plane(Fb, k, sz);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            c_xx[r] = mld<T, C>(pxx, t.ro[r], t.rin[r], fi_xx);
-            c_xy[r] = mld<T, C>(pxy, t.ro[r], t.rin[r], fi_xy);
-            c_yy[r] = mld<T, C>(pyy, t.ro[r], t.rin[r], fi_yy);
-            h_xx[r] = mld<T, C>(pxx, t.hro[r], t.hok[r], fi_xx);
-            h_xy[r] = mld<T, C>(pxy, t.hro[r], t.hok[r], fi_xy);
-            h_xz[r] = mld<T, C>(pxz, t.hro[r], t.hok[r], fi_xz);
-            b_x[r] = mld<T, C>(pb, t.bxo[r], t.bxok && t.rin[r], ib);
+            c_xx[r] = G::ld(pxx, t.ro[r], t.rin[r], fi_xx);
+            c_xy[r] = G::ld(pxy, t.ro[r], t.rin[r], fi_xy);
+            c_yy[r] = G::ld(pyy, t.ro[r], t.rin[r], fi_yy);
+            h_xx[r] = G::ld(pxx, t.hro[r], t.hok[r], fi_xx);
+            h_xy[r] = G::ld(pxy, t.hro[r], t.hok[r], fi_xy);
+            h_xz[r] = G::ld(pxz, t.hro[r], t.hok[r], fi_xz);
+            b_x[r] = G::ld(pb, t.bxo[r], t.bxok && t.rin[r], ib);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            y_yy[m] = mld<T, C>(pyy, t.yro[m], t.yok[m], fi_yy);
-            y_yz[m] = mld<T, C>(pyz, t.yro[m], t.yok[m], fi_yz);
-            y_xy[m] = mld<T, C>(pxy, t.yro[m], t.yok[m], fi_xy);
+            y_yy[m] = G::ld(pyy, t.yro[m], t.yok[m], fi_yy);
+            y_yz[m] = G::ld(pyz, t.yro[m], t.yok[m], fi_yz);
+            y_xy[m] = G::ld(pxy, t.yro[m], t.yok[m], fi_xy);
         }
-        b_y = mld<T, C>(pb, t.byo, t.byok, ib);
+        b_y = G::ld(pb, t.byo, t.byok, ib);
     };
     fetch(k0);
 
-    // old velocities of plane kk (+ compensation residuals)
-    auto load_rmw = [&](int kk, C (&out)[3][NY]) {
+    // old velocities of plane kk (stored values; compensated kernels also the
+    // residuals), converted by rmw_value where the update uses them
+    auto load_rmw = [&](int kk, T (&out)[3][NY], T (&res)[3][NY]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T *p = plane(s.f[VX + c], kk, sz);
 #pragma unroll
-            for (int r = 0; r < NY; ++r) out[c][r] = Cvt<T, C>::ld(bld_nt(p, t.ro[r]), s.finv[VX + c]);
+            for (int r = 0; r < NY; ++r) out[c][r] = bld_nt(p, t.ro[r]);
         }
         if (CP) {
-            // compensated update: the old value is stored value + residual
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 if (s.comp[VX + c]) {
                     const T *pc = plane(s.comp[VX + c], kk, sz);
 #pragma unroll
-                    for (int r = 0; r < NY; ++r) out[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[VX + c]);
+                    for (int r = 0; r < NY; ++r) res[c][r] = bld_nt(pc, t.ro[r]);
                 }
         }
+    };
+    // compensated update: the old value is stored value + residual
+    auto rmw_value = [&](const T (&raw)[3][NY], const T (&res)[3][NY], int c, int r) -> C {
+        C v = Cvt<T, C>::ld(raw[c][r], s.finv[VX + c]);
+        if (CP && s.comp[VX + c]) v += Cvt<T, C>::ld(res[c][r], s.cinv[VX + c]);
+        return v;
     };
     // the RMW operands are prefetched one plane ahead in interior boxes, and in
     // every box with 16-bit storage (measured: +4 % FP16, -4 % FP32 in PML boxes)
     constexpr bool RP = DM == 0 || sizeof(T) == 2;
-    C vpre[3][NY];
-    if (RP) load_rmw(k0, vpre);
+    T vpre[3][NY], vpres[3][NY];
+    if (RP) load_rmw(k0, vpre, vpres);
 
     // damping: x per lane (edge tiles), y per box (y cuts at the PML planes),
     // z per plane, each only if the box's mask DM allows that axis; every
@@ -558,23 +599,33 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 
     for (int k = k0; k < k1; ++k) {
         auto &L = vplane(Ls, k);
+        // staged values of plane k (loaded during plane k-1)
+        C cxx[NY], cxy[NY], hxx[NY], hxy[NY], hxz[NY];
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            cxx[r] = G::use(c_xx[r], t.rin[r], fi_xx);
+            cxy[r] = G::use(c_xy[r], t.rin[r], fi_xy);
+            hxx[r] = G::use(h_xx[r], t.hok[r], fi_xx);
+            hxy[r] = G::use(h_xy[r], t.hok[r], fi_xy);
+            hxz[r] = G::use(h_xz[r], t.hok[r], fi_xz);
+        }
         // x derivatives across lanes, before the staging registers are reused
         C dxl[3][NY];
         if (XL) {
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                dxl[0][r] = xfwd(c_xx[r], h_xx[r], s.cx);
-                dxl[1][r] = xbwd(c_xy[r], h_xy[r], s.cx);
-                dxl[2][r] = xbwd(qxz[r][5], h_xz[r], s.cx);
+                dxl[0][r] = xfwd(cxx[r], hxx[r], s.cx);
+                dxl[1][r] = xbwd(cxy[r], hxy[r], s.cx);
+                dxl[2][r] = xbwd(qxz[r][5], hxz[r], s.cx);
             }
         }
         // ---- stage plane k
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
-            if (!XL) L.sxx[lr][tx + 5] = c_xx[r];
-            L.sxy[lr + 5][tx + 5] = c_xy[r];
-            L.syy[lr + 5][tx] = c_yy[r];
+            if (!XL) L.sxx[lr][tx + 5] = cxx[r];
+            L.sxy[lr + 5][tx + 5] = cxy[r];
+            L.syy[lr + 5][tx] = G::use(c_yy[r], t.rin[r], fi_yy);
             if (!XL) L.sxz[lr][tx + 5] = qxz[r][5];
             L.syz[lr + 5][tx] = qyz[r][5];
             L.b[lr][tx] = qb[r][0];
@@ -583,29 +634,29 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 const int lr = ty * NY + r;
-                L.sxx[lr][t.hc] = h_xx[r];
-                L.sxy[lr + 5][t.hc] = h_xy[r];
-                L.sxz[lr][t.hc] = h_xz[r];
+                L.sxx[lr][t.hc] = hxx[r];
+                L.sxy[lr + 5][t.hc] = hxy[r];
+                L.sxz[lr][t.hc] = hxz[r];
             }
         }
         if (tx == 10) {
 #pragma unroll
-            for (int r = 0; r < NY; ++r) L.b[ty * NY + r][TX] = b_x[r];
+            for (int r = 0; r < NY; ++r) L.b[ty * NY + r][TX] = G::use(b_x[r], t.bxok && t.rin[r], ib);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
             if (ty + BY * m < 10) {
-                L.syy[t.ylr[m]][tx] = y_yy[m];
-                L.syz[t.ylr[m]][tx] = y_yz[m];
-                L.sxy[t.ylr[m]][tx + 5] = y_xy[m];
+                L.syy[t.ylr[m]][tx] = G::use(y_yy[m], t.yok[m], fi_yy);
+                L.syz[t.ylr[m]][tx] = G::use(y_yz[m], t.yok[m], fi_yz);
+                L.sxy[t.ylr[m]][tx + 5] = G::use(y_xy[m], t.yok[m], fi_xy);
             }
         }
-        if (ty == 0) L.b[TY][tx] = b_y;
+        if (ty == 0) L.b[TY][tx] = G::use(b_y, t.byok, ib);
         __syncthreads();
 
         // ---- read half of the read-modify-write of plane k: loaded one plane
         // ahead (RP); else issued first, so waiting for it does not wait for
-        // the prefetch below
+        // the loads below
         T *pv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) pv[c] = plane(s.f[VX + c], k, sz);
@@ -614,9 +665,14 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 #pragma unroll
             for (int r = 0; r < NY; ++r)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) vold[c][r] = vpre[c][r];
+                for (int c = 0; c < 3; ++c) vold[c][r] = rmw_value(vpre, vpres, c, r);
         } else {
-            load_rmw(k, vold);
+            T vr[3][NY], vrs[3][NY];
+            load_rmw(k, vr, vrs);
+#pragma unroll
+            for (int r = 0; r < NY; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) vold[c][r] = rmw_value(vr, vrs, c, r);
         }
         bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1, so[NY][3][3], vsr[NY];
@@ -653,24 +709,31 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
             }
         }
 
-        // ---- loads for later: plane k+1 staging and RMW operands, queue heads
-        if (k + 1 < k1) {
-            fetch(k + 1);
-            if (RP) load_rmw(k + 1, vpre);
-        }
-        C nxz[NY], nyz[NY], nzz[NY], nb[NY];
+        // ---- loads for later, in the order their values are needed: the RMW
+        // operands of plane k+1 (their registers are copied when the loop
+        // wraps) and the queue heads (pushed after this plane) before the
+        // staging values of plane k+1 (first used after the next barrier).
+        // The vector-memory counter retires in issue order, so the waits
+        // for the first two groups do not include the third.  The last
+        // plane of the box reloads itself (just read, cached): with no branch
+        // around these loads the compiler's waits count them in every
+        // iteration.
+        const int kn = k + 1 < k1 ? k + 1 : k;
+        if (RP) load_rmw(kn, vpre, vpres);
+        S nxz[NY], nyz[NY], nzz[NY], nb[NY];
+        const bool z5 = k + 5 < g.nz, z6 = k + 6 < g.nz, z2 = k + 2 < g.nz;
         {
-            const bool z5 = k + 5 < g.nz, z6 = k + 6 < g.nz, z2 = k + 2 < g.nz;
             const T *pxz = plane(Fxz, z5 ? k + 5 : k, sz), *pyz = plane(Fyz, z5 ? k + 5 : k, sz);
             const T *pzz = plane(Fzz, z6 ? k + 6 : k, sz), *pb = plane(Fb, z2 ? k + 2 : k, sz);
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                nxz[r] = mld<T, C>(pxz, t.ro[r], t.rin[r] && z5, fi_xz);
-                nyz[r] = mld<T, C>(pyz, t.ro[r], t.rin[r] && z5, fi_yz);
-                nzz[r] = mld<T, C>(pzz, t.ro[r], t.rin[r] && z6, fi_zz);
-                nb[r] = mld<T, C>(pb, t.ro[r], t.rin[r] && z2, ib);
+                nxz[r] = G::ld(pxz, t.ro[r], t.rin[r] && z5, fi_xz);
+                nyz[r] = G::ld(pyz, t.ro[r], t.rin[r] && z5, fi_yz);
+                nzz[r] = G::ld(pzz, t.ro[r], t.rin[r] && z6, fi_zz);
+                nb[r] = G::ld(pb, t.ro[r], t.rin[r] && z2, ib);
             }
         }
+        fetch(kn);
 
         // ---- compute plane k
 #pragma unroll
@@ -751,11 +814,11 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 qyz[r][q] = qyz[r][q + 1];
                 qzz[r][q] = qzz[r][q + 1];
             }
-            qxz[r][9] = nxz[r];
-            qyz[r][9] = nyz[r];
-            qzz[r][9] = nzz[r];
+            qxz[r][9] = G::use(nxz[r], t.rin[r] && z5, fi_xz);
+            qyz[r][9] = G::use(nyz[r], t.rin[r] && z5, fi_yz);
+            qzz[r][9] = G::use(nzz[r], t.rin[r] && z6, fi_zz);
             qb[r][0] = qb[r][1];
-            qb[r][1] = nb[r];
+            qb[r][1] = G::use(nb[r], t.rin[r] && z2, ib);
         }
     }
 }
@@ -796,71 +859,78 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     }
 
     // mu staging registers (center rows, x halo, y halo, corner) of one plane
-    C m_c[NY], m_x[NY], m_y, m_xy;
+    // (stored values, converted when staged)
+    using G = Stg<T, C>;
+    using S = typename G::S;
+    S m_c[NY], m_x[NY], m_y, m_xy;
     auto fetch_mu = [&](int k) {
         const T *pm = plane(Fmu, k, sz);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            m_c[r] = mld<T, C>(pm, t.ro[r], t.rin[r], im);
-            m_x[r] = mld<T, C>(pm, t.bxo[r], t.bxok && t.rin[r], im);
+            m_c[r] = G::ld(pm, t.ro[r], t.rin[r], im);
+            m_x[r] = G::ld(pm, t.bxo[r], t.bxok && t.rin[r], im);
         }
-        m_y = mld<T, C>(pm, t.byo, t.byok, im);
-        m_xy = mld<T, C>(pm, t.bxyo, t.bxyok, im);
+        m_y = G::ld(pm, t.byo, t.byok, im);
+        m_xy = G::ld(pm, t.bxyo, t.bxyok, im);
     };
     // reciprocals into the ring slot of plane k (0 outside the array: never
-    // read by an updated point)
-    auto stage_mu = [&](int k) {
+    // read by an updated point); mc: mu of the centre rows
+    auto stage_mu = [&](int k, C (&mc)[NY]) {
         auto &R = rmslot(Ls, k);
 #pragma unroll
-        for (int r = 0; r < NY; ++r) R[ty * NY + r][tx] = t.rin[r] ? (C)1 / m_c[r] : (C)0;
+        for (int r = 0; r < NY; ++r) {
+            mc[r] = G::use(m_c[r], t.rin[r], im);
+            R[ty * NY + r][tx] = t.rin[r] ? (C)1 / mc[r] : (C)0;
+        }
         if (tx == 10) {
 #pragma unroll
-            for (int r = 0; r < NY; ++r) R[ty * NY + r][TX] = (t.bxok && t.rin[r]) ? (C)1 / m_x[r] : (C)0;
+            for (int r = 0; r < NY; ++r) {
+                const bool ok = t.bxok && t.rin[r];
+                R[ty * NY + r][TX] = ok ? (C)1 / G::use(m_x[r], ok, im) : (C)0;
+            }
         }
         if (ty == 0) {
-            R[TY][tx] = t.byok ? (C)1 / m_y : (C)0;
-            if (tx == 10) R[TY][TX] = t.bxyok ? (C)1 / m_xy : (C)0;
+            R[TY][tx] = t.byok ? (C)1 / G::use(m_y, t.byok, im) : (C)0;
+            if (tx == 10) R[TY][TX] = t.bxyok ? (C)1 / G::use(m_xy, t.bxyok, im) : (C)0;
         }
     };
     C muk[NY], mun[NY];  // mu of planes k and k+1 (for lambda + 2 mu)
     fetch_mu(k0);
-#pragma unroll
-    for (int r = 0; r < NY; ++r) muk[r] = m_c[r];
-    stage_mu(k0);
+    stage_mu(k0, muk);
     fetch_mu(k0 + 1);
-#pragma unroll
-    for (int r = 0; r < NY; ++r) mun[r] = m_c[r];
-    stage_mu(k0 + 1);
+    stage_mu(k0 + 1, mun);
 
-    // staging registers: x and y halos of the three velocities
-    C h_vx[NY], h_vy[NY], h_vz[NY], y_vx[NH], y_vy[NH], y_vz[NH];
+    // staging registers: x and y halos of the three velocities (stored
+    // values, converted and masked when staged)
+    S h_vx[NY], h_vy[NY], h_vz[NY], y_vx[NH], y_vy[NH], y_vz[NH];
     auto fetch = [&](int k) {
         const T *pvx = plane(Fvx, k, sz), *pvy = plane(Fvy, k, sz), *pvz = plane(Fvz, k, sz);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            h_vx[r] = mld<T, C>(pvx, t.hro[r], t.hok[r], fi_vx);
-            h_vy[r] = mld<T, C>(pvy, t.hro[r], t.hok[r], fi_vy);
-            h_vz[r] = mld<T, C>(pvz, t.hro[r], t.hok[r], fi_vz);
+            h_vx[r] = G::ld(pvx, t.hro[r], t.hok[r], fi_vx);
+            h_vy[r] = G::ld(pvy, t.hro[r], t.hok[r], fi_vy);
+            h_vz[r] = G::ld(pvz, t.hro[r], t.hok[r], fi_vz);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            y_vx[m] = mld<T, C>(pvx, t.yro[m], t.yok[m], fi_vx);
-            y_vy[m] = mld<T, C>(pvy, t.yro[m], t.yok[m], fi_vy);
-            y_vz[m] = mld<T, C>(pvz, t.yro[m], t.yok[m], fi_vz);
+            y_vx[m] = G::ld(pvx, t.yro[m], t.yok[m], fi_vx);
+            y_vy[m] = G::ld(pvy, t.yro[m], t.yok[m], fi_vy);
+            y_vz[m] = G::ld(pvz, t.yro[m], t.yok[m], fi_vz);
         }
     };
     fetch(k0);
 
-    // old stresses and lambda of plane kk (+ compensation residuals)
-    auto load_rmw = [&](int kk, C (&out)[6][NY], C (&lo)[NY]) {
+    // old stresses and lambda of plane kk (stored values; compensated kernels
+    // also the residuals), converted by rmw_value where the update uses them
+    auto load_rmw = [&](int kk, T (&out)[6][NY], T (&res)[6][NY], T (&lo)[NY]) {
         const T *pl = plane(Flam, kk, sz);
 #pragma unroll
-        for (int r = 0; r < NY; ++r) lo[r] = Cvt<T, C>::ld(bld_nt(pl, t.ro[r]), il);
+        for (int r = 0; r < NY; ++r) lo[r] = bld_nt(pl, t.ro[r]);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             const T *p = plane(s.f[SXX + c], kk, sz);
 #pragma unroll
-            for (int r = 0; r < NY; ++r) out[c][r] = Cvt<T, C>::ld(bld_nt(p, t.ro[r]), s.finv[SXX + c]);
+            for (int r = 0; r < NY; ++r) out[c][r] = bld_nt(p, t.ro[r]);
         }
         if (CP) {
 #pragma unroll
@@ -868,15 +938,21 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 if (s.comp[SXX + c]) {
                     const T *pc = plane(s.comp[SXX + c], kk, sz);
 #pragma unroll
-                    for (int r = 0; r < NY; ++r) out[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[SXX + c]);
+                    for (int r = 0; r < NY; ++r) res[c][r] = bld_nt(pc, t.ro[r]);
                 }
         }
+    };
+    // compensated update: the old value is stored value + residual
+    auto rmw_value = [&](const T (&raw)[6][NY], const T (&res)[6][NY], int c, int r) -> C {
+        C v = Cvt<T, C>::ld(raw[c][r], s.finv[SXX + c]);
+        if (CP && s.comp[SXX + c]) v += Cvt<T, C>::ld(res[c][r], s.cinv[SXX + c]);
+        return v;
     };
     // the RMW operands are prefetched one plane ahead in interior boxes, and in
     // every box with 16-bit storage (measured: +4 % FP16, -4 % FP32 in PML boxes)
     constexpr bool RP = DM == 0 || sizeof(T) == 2;
-    C spre[6][NY], lpre[NY];
-    if (RP) load_rmw(k0, spre, lpre);
+    T spre[6][NY], spres[6][NY], lpre[NY];
+    if (RP) load_rmw(k0, spre, spres, lpre);
 
     // damping as in the velocity sweep; splits per component: XX, YY, ZZ:
     // x, y, z; XY: x, y; XZ: x, z; YZ: y, z
@@ -906,9 +982,9 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         if (XL) {
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                dxl[0][r] = xbwd(qvx[r][4], h_vx[r], s.cx);
-                dxl[1][r] = xfwd(qvy[r][4], h_vy[r], s.cx);
-                dxl[2][r] = xfwd(qvz[r][5], h_vz[r], s.cx);
+                dxl[0][r] = xbwd(qvx[r][4], G::use(h_vx[r], t.hok[r], fi_vx), s.cx);
+                dxl[1][r] = xfwd(qvy[r][4], G::use(h_vy[r], t.hok[r], fi_vy), s.cx);
+                dxl[2][r] = xfwd(qvz[r][5], G::use(h_vz[r], t.hok[r], fi_vz), s.cx);
             }
         }
 #pragma unroll
@@ -922,17 +998,17 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 const int lr = ty * NY + r;
-                L.vx[lr + 5][t.hc] = h_vx[r];
-                L.vy[lr + 5][t.hc] = h_vy[r];
-                L.vz[lr + 5][t.hc] = h_vz[r];
+                L.vx[lr + 5][t.hc] = G::use(h_vx[r], t.hok[r], fi_vx);
+                L.vy[lr + 5][t.hc] = G::use(h_vy[r], t.hok[r], fi_vy);
+                L.vz[lr + 5][t.hc] = G::use(h_vz[r], t.hok[r], fi_vz);
             }
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
             if (ty + BY * m < 10) {
-                L.vx[t.ylr[m]][tx + 5] = y_vx[m];
-                L.vy[t.ylr[m]][tx + 5] = y_vy[m];
-                L.vz[t.ylr[m]][tx + 5] = y_vz[m];
+                L.vx[t.ylr[m]][tx + 5] = G::use(y_vx[m], t.yok[m], fi_vx);
+                L.vy[t.ylr[m]][tx + 5] = G::use(y_vy[m], t.yok[m], fi_vy);
+                L.vz[t.ylr[m]][tx + 5] = G::use(y_vz[m], t.yok[m], fi_vz);
             }
         }
         __syncthreads();
@@ -947,12 +1023,19 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         if (RP) {
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                lam[r] = lpre[r];
+                lam[r] = Cvt<T, C>::ld(lpre[r], il);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) old[c][r] = spre[c][r];
+                for (int c = 0; c < 6; ++c) old[c][r] = rmw_value(spre, spres, c, r);
             }
         } else {
-            load_rmw(k, old, lam);
+            T sr[6][NY], srs[6][NY], lr_[NY];
+            load_rmw(k, sr, srs, lr_);
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                lam[r] = Cvt<T, C>::ld(lr_[r], il);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) old[c][r] = rmw_value(sr, srs, c, r);
+            }
         }
         if (k == src_k && s.src_saved) {
             // TRANSIENT source: the stress update starts from sigma before
@@ -1014,25 +1097,31 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
             }
         }
 
-        // ---- loads for later
+        // ---- loads for later.  First those used after this plane (queue
+        // heads, mu of plane k+2): the vector-memory counter retires in issue
+        // order, so waiting for them does not wait for plane k+1's staging
+        // and RMW operands issued after them (first used after the next
+        // barrier).
         const bool more = k + 1 < k1;
-        if (more) {
-            fetch(k + 1);
-            if (RP) load_rmw(k + 1, spre, lpre);
-            fetch_mu(k + 2);  // staged into the ring after this plane
-        }
-        C nvx[NY], nvy[NY], nvz[NY];
+        S nvx[NY], nvy[NY], nvz[NY];
+        const bool z6 = k + 6 < g.nz, z5 = k + 5 < g.nz;
         {
-            const bool z6 = k + 6 < g.nz, z5 = k + 5 < g.nz;
             const T *px_ = plane(Fvx, z6 ? k + 6 : k, sz), *py_ = plane(Fvy, z6 ? k + 6 : k, sz);
             const T *pz_ = plane(Fvz, z5 ? k + 5 : k, sz);
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                nvx[r] = mld<T, C>(px_, t.ro[r], t.rin[r] && z6, fi_vx);
-                nvy[r] = mld<T, C>(py_, t.ro[r], t.rin[r] && z6, fi_vy);
-                nvz[r] = mld<T, C>(pz_, t.ro[r], t.rin[r] && z5, fi_vz);
+                nvx[r] = G::ld(px_, t.ro[r], t.rin[r] && z6, fi_vx);
+                nvy[r] = G::ld(py_, t.ro[r], t.rin[r] && z6, fi_vy);
+                nvz[r] = G::ld(pz_, t.ro[r], t.rin[r] && z5, fi_vz);
             }
         }
+        // the last plane of the box reloads planes just read (cached) instead
+        // of skipping the loads: with no branch around them the compiler's
+        // wait for the queue heads counts them in every iteration
+        const int kn = more ? k + 1 : k;
+        fetch_mu(kn + 1);  // staged into the ring after this plane
+        fetch(kn);
+        if (RP) load_rmw(kn, spre, spres, lpre);
 
         auto &R0 = rmslot(Ls, k);
         auto &R1 = rmslot(Ls, k + 1);
@@ -1144,11 +1233,12 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 
         // ---- ring and queues advance
         if (more) {
-            stage_mu(k + 2);
+            C m2[NY];
+            stage_mu(k + 2, m2);
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 muk[r] = mun[r];
-                mun[r] = m_c[r];
+                mun[r] = m2[r];
             }
         }
 #pragma unroll
@@ -1159,9 +1249,9 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
                 qvy[r][q] = qvy[r][q + 1];
                 qvz[r][q] = qvz[r][q + 1];
             }
-            qvx[r][9] = nvx[r];
-            qvy[r][9] = nvy[r];
-            qvz[r][9] = nvz[r];
+            qvx[r][9] = G::use(nvx[r], t.rin[r] && z6, fi_vx);
+            qvy[r][9] = G::use(nvy[r], t.rin[r] && z6, fi_vy);
+            qvz[r][9] = G::use(nvz[r], t.rin[r] && z5, fi_vz);
         }
     }
 }
