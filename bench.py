@@ -302,7 +302,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="layered512")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16", "bf16"])
-    ap.add_argument("--kernel", default="tiled", choices=["tiled", "simple"])
+    ap.add_argument("--kernel", default=None, choices=["tiled", "simple"],
+                    help="default: tiled (FP64 runs the per-point kernels)")
     ap.add_argument("--tile", type=int, nargs=5, default=None, metavar=("TY", "NY", "PTY", "PNY", "ZCHUNK"),
                     help="interior box rows and rows per thread, PML box rows and rows per thread, "
                          "z planes per box (0 = auto)")
@@ -375,7 +376,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.description, "grid": [wl.nx, wl.ny, wl.nz], "pml": wl.pml,
                    "points": {"total": wl.points, "updated": upd, "interior": inner},
-                   "precision": args.precision, "kernel": args.kernel, "parallelism": "single-gpu"},
+                   "precision": args.precision, "kernel": kernel_of(args), "parallelism": "single-gpu"},
         "roofline": roof,
         "sweeps_ms": r["sweeps_ms"],
         "launch": r["launch"],
@@ -401,6 +402,13 @@ def main():
     print(json.dumps(out))
 
 
+def kernel_of(args, precision=None):
+    """The sweep kernels a run uses: --kernel, else the tiled sweeps (the
+    per-point kernels for FP64, the only ones the library runs in FP64)."""
+    p = precision or args.precision
+    return args.kernel or ("simple" if p == "fp64" else "tiled")
+
+
 DTYPES = {"fp32": "f32", "fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}
 
 
@@ -414,7 +422,7 @@ def timed_run(sgfd, wl, args, precision):
         if args.streams:
             s.set_streams(args.streams)
         launch = None
-        if args.tune and args.kernel == "tiled":
+        if args.tune and kernel_of(args, precision) == "tiled":
             log(f"bench: {precision} autotune")
             tile, tms = s.autotune(4)
             launch = {"tile": list(tile), "streams": s.streams(), "autotune_ms_per_step": tms}

@@ -82,7 +82,7 @@ public:
         c.time_length = timeLength;
         c.precision = precision;
         c.source_mode = source_mode;
-        c.kernel = SG_KERNEL_TILED;
+        c.kernel = precision == SG_FP64 ? SG_KERNEL_SIMPLE : SG_KERNEL_TILED;  // FP64: per-point kernels
         sg::check(sg_create(&c, &s_));
         sg::check(sg_set_medium(s_, medCPU.buoyancy, medCPU.lambda, medCPU.mu));
         std::vector<float> w(timeLength);

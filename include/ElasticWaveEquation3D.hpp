@@ -75,7 +75,7 @@ public:
         c.time_length = timeLength;
         c.precision = precision;
         c.source_mode = SG_SOURCE_TRANSIENT;
-        c.kernel = SG_KERNEL_TILED;
+        c.kernel = precision == SG_FP64 ? SG_KERNEL_SIMPLE : SG_KERNEL_TILED;  // FP64: per-point kernels
         sg::check(sg_create(&c, &s_));
         try {
             // R, tau and Vs are float constants in the reference

@@ -114,7 +114,9 @@ typedef struct sg_config {
 
 /* Allocates device state (fields, PML splits, tables) on the current HIP
  * device.  Fields start at zero; damping defaults to the reference CPU
- * formula (tau 3.4, V 2000, R 1e-3) until sg_set_damping is called. */
+ * formula (tau 3.4, V 2000, R 1e-3) until sg_set_damping is called.
+ * SG_FP64 runs SG_KERNEL_SIMPLE only (SG_ERR_UNSUPPORTED for the tiled
+ * sweeps). */
 int sg_create(const sg_config *cfg, sg_solver **out);
 /* Frees the solver (NULL: no-op).  SG_ERR_STATE, and nothing freed, while a
  * batch holds it: destroy the batch first. */

@@ -508,6 +508,12 @@ int sg_create(const sg_config *cfg, sg_solver **out)
         return fail(SG_ERR_INVALID, "sg_create: bad source_mode");
     if (c.kernel != SG_KERNEL_TILED && c.kernel != SG_KERNEL_SIMPLE)
         return fail(SG_ERR_INVALID, "sg_create: bad kernel");
+    // FP64 is the checking build and runs the per-point kernels: its tiled
+    // sweeps (256 VGPRs plus AGPR and SGPR spills) took GPU memory faults on
+    // small grids whose cause is not found, so they are not offered
+    if (c.kernel == SG_KERNEL_TILED && c.precision == SG_FP64)
+        return fail(SG_ERR_UNSUPPORTED, "sg_create: FP64 runs the per-point kernels (SG_KERNEL_SIMPLE); "
+                                        "the tiled FP64 sweeps are disabled");
     const long long N = (long long)c.nx * c.ny * c.nz;
     // indices are 64-bit; the tiled sweeps address inside one xy plane with
     // 32-bit byte offsets

@@ -440,9 +440,12 @@ __device__ __forceinline__ void tile_geo(const Geom &g, int i0, int i1, int j0, 
         t.ro[r] = (unsigned)(t.ic + jc * sy) * ES;
         t.rin[r] = xin && yin;
         t.rup[r] = xup && j < j1;
-        t.hro[r] = (unsigned)(hic + jc * sy) * ES;
         t.hok[r] = hx && yin;
-        t.bxo[r] = (unsigned)(bxc + jc * sy) * ES;
+        // a lane whose value is masked reads its own centre instead: a cache
+        // line the wave reads anyway, so a halo load touches no other lines
+        // than the halo's own
+        t.hro[r] = t.hok[r] ? (unsigned)(hic + jc * sy) * ES : t.ro[r];
+        t.bxo[r] = (t.bxok && t.rin[r]) ? (unsigned)(bxc + jc * sy) * ES : t.ro[r];
     }
 #pragma unroll
     for (int m = 0; m < TileGeo<TY, NY>::NH; ++m) {
@@ -451,14 +454,14 @@ __device__ __forceinline__ void tile_geo(const Geom &g, int i0, int i1, int j0, 
         const int jh = j0 - HALO + lr;
         t.ylr[m] = lr;
         t.yok[m] = h < 10 && xin && jh < g.ny;
-        t.yro[m] = (unsigned)(t.ic + (jh < g.ny ? jh : g.ny - 1) * sy) * ES;
+        t.yro[m] = t.yok[m] ? (unsigned)(t.ic + jh * sy) * ES : t.ro[0];
     }
     const int byj = j0 + TY;
     const int byc = byj < g.ny ? byj : g.ny - 1;
     t.byok = ty == 0 && xin && byj < g.ny;
     t.bxyok = ty == 0 && tx == 10 && bxi < g.nx && byj < g.ny;
-    t.byo = (unsigned)(t.ic + byc * sy) * ES;
-    t.bxyo = (unsigned)(bxc + byc * sy) * ES;
+    t.byo = t.byok ? (unsigned)(t.ic + byc * sy) * ES : t.ro[0];
+    t.bxyo = t.bxyok ? (unsigned)(bxc + byc * sy) * ES : t.ro[0];
 }
 
 // Store of an updated value with compensation when field f has a
@@ -1287,8 +1290,9 @@ namespace sg {
 template <class T, int DM, int W, int TY, int NY>
 struct LeadOn {
     static constexpr int bit = DM == 0 ? 0 : (DM == 1 ? 1 : (DM == 2 ? 2 : (DM == 4 ? 3 : 4)));
-    static constexpr int mask =
-        sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16) : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
+    static constexpr int mask = sizeof(T) == 8 ? 0
+                                : sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16)
+                                                 : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
     static constexpr bool v = ((mask >> bit) & 1) != 0 && TY == 4 && NY == 1;
 };
 
@@ -1465,8 +1469,9 @@ __device__ __forceinline__ int list_of_item(const ListOffsets &lo, int p)
 // LDS of a fused launch: large enough for every list's form
 template <class T, class C, int W>
 struct FusedLds {
-    static constexpr int mask = sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16)
-                                               : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
+    static constexpr int mask = sizeof(T) == 8 ? 0
+                                : sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16)
+                                                 : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
     using type = typename std::conditional<
         W == 0,
         typename std::conditional<(mask & 0x1F) != 0, VelLdsL<C, 4>,

@@ -49,6 +49,8 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
     const long long sz = g.sz;
     TileGeo<TY, NY> t;
     tile_geo<TY, NY, (int)sizeof(T)>(g, i0, i1, j0, j1, t);
+    using G = Stg<T, C>;
+    using S = typename G::S;
 
     const T *Fxx = s.f[SXX], *Fyy = s.f[SYY], *Fzz = s.f[SZZ];
     const T *Fxy = s.f[SXY], *Fxz = s.f[SXZ], *Fyz = s.f[SYZ];
@@ -81,16 +83,17 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
         qb[r][1] = mld<T, C>(plane(Fb, k0 + 1, sz), t.ro[r], t.rin[r], ib);
     }
 
-    // lead-plane staging: x halo of sxz, y halo of syz
+    // lead-plane staging: x halo of sxz, y halo of syz (staging registers,
+    // converted when staged)
     struct Halo {
-        C xz[NY], yz[NH];
+        S xz[NY], yz[NH];
     };
     auto fetch_lead = [&](int k, Halo &h) {
         const T *pxz = plane(Fxz, k, sz), *pyz = plane(Fyz, k, sz);
 #pragma unroll
-        for (int r = 0; r < NY; ++r) h.xz[r] = mld<T, C>(pxz, t.hro[r], t.hok[r], fi_xz);
+        for (int r = 0; r < NY; ++r) h.xz[r] = G::ld(pxz, t.hro[r], t.hok[r], fi_xz);
 #pragma unroll
-        for (int m = 0; m < NH; ++m) h.yz[m] = mld<T, C>(pyz, t.yro[m], t.yok[m], fi_yz);
+        for (int m = 0; m < NH; ++m) h.yz[m] = G::ld(pyz, t.yro[m], t.yok[m], fi_yz);
     };
     // centres of the lead plane come from the queues (index qi)
     auto stage_lead = [&](int qi, const Halo &h) {
@@ -102,11 +105,11 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
         }
         if (tx < 10) {
 #pragma unroll
-            for (int r = 0; r < NY; ++r) L.sxz[ty * NY + r][t.hc] = h.xz[r];
+            for (int r = 0; r < NY; ++r) L.sxz[ty * NY + r][t.hc] = G::use(h.xz[r], t.hok[r], fi_xz);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m)
-            if (ty + BY * m < 10) L.syz[t.ylr[m]][tx] = h.yz[m];
+            if (ty + BY * m < 10) L.syz[t.ylr[m]][tx] = G::use(h.yz[m], t.yok[m], fi_yz);
     };
     // D-x sxz and D-y syz of the staged lead plane, into ring slot sl
     const int tid = ty * TX + tx;
@@ -135,36 +138,37 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
         }
     }
 
-    // updated-plane staging registers
-    C c_xx[NY], c_xy[NY], c_yy[NY], h_xx[NY], h_xy[NY], b_x[NY];
-    C y_yy[NH], y_xy[NH], b_y;
+    // updated-plane staging registers (converted when staged)
+    S c_xx[NY], c_xy[NY], c_yy[NY], h_xx[NY], h_xy[NY], b_x[NY];
+    S y_yy[NH], y_xy[NH], b_y;
     auto fetch = [&](int k) {
         const T *pxx = plane(Fxx, k, sz), *pxy = plane(Fxy, k, sz), *pyy = plane(Fyy, k, sz);
         const T *pb = plane(Fb, k, sz);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            c_xx[r] = mld<T, C>(pxx, t.ro[r], t.rin[r], fi_xx);
-            c_xy[r] = mld<T, C>(pxy, t.ro[r], t.rin[r], fi_xy);
-            c_yy[r] = mld<T, C>(pyy, t.ro[r], t.rin[r], fi_yy);
-            h_xx[r] = mld<T, C>(pxx, t.hro[r], t.hok[r], fi_xx);
-            h_xy[r] = mld<T, C>(pxy, t.hro[r], t.hok[r], fi_xy);
-            b_x[r] = mld<T, C>(pb, t.bxo[r], t.bxok && t.rin[r], ib);
+            c_xx[r] = G::ld(pxx, t.ro[r], t.rin[r], fi_xx);
+            c_xy[r] = G::ld(pxy, t.ro[r], t.rin[r], fi_xy);
+            c_yy[r] = G::ld(pyy, t.ro[r], t.rin[r], fi_yy);
+            h_xx[r] = G::ld(pxx, t.hro[r], t.hok[r], fi_xx);
+            h_xy[r] = G::ld(pxy, t.hro[r], t.hok[r], fi_xy);
+            b_x[r] = G::ld(pb, t.bxo[r], t.bxok && t.rin[r], ib);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            y_yy[m] = mld<T, C>(pyy, t.yro[m], t.yok[m], fi_yy);
-            y_xy[m] = mld<T, C>(pxy, t.yro[m], t.yok[m], fi_xy);
+            y_yy[m] = G::ld(pyy, t.yro[m], t.yok[m], fi_yy);
+            y_xy[m] = G::ld(pxy, t.yro[m], t.yok[m], fi_xy);
         }
-        b_y = mld<T, C>(pb, t.byo, t.byok, ib);
+        b_y = G::ld(pb, t.byo, t.byok, ib);
     };
     fetch(k0);
 
-    auto load_rmw = [&](int kk, C (&out)[3][NY]) {
+    // old velocities (stored values and residuals), converted where used
+    auto load_rmw = [&](int kk, T (&out)[3][NY], T (&res)[3][NY]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T *p = plane(s.f[VX + c], kk, sz);
 #pragma unroll
-            for (int r = 0; r < NY; ++r) out[c][r] = Cvt<T, C>::ld(bld_nt(p, t.ro[r]), s.finv[VX + c]);
+            for (int r = 0; r < NY; ++r) out[c][r] = bld_nt(p, t.ro[r]);
         }
         if (CP) {
 #pragma unroll
@@ -172,13 +176,18 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
                 if (s.comp[VX + c]) {
                     const T *pc = plane(s.comp[VX + c], kk, sz);
 #pragma unroll
-                    for (int r = 0; r < NY; ++r) out[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[VX + c]);
+                    for (int r = 0; r < NY; ++r) res[c][r] = bld_nt(pc, t.ro[r]);
                 }
         }
     };
+    auto rmw_value = [&](const T (&raw)[3][NY], const T (&res)[3][NY], int c, int r) -> C {
+        C v = Cvt<T, C>::ld(raw[c][r], s.finv[VX + c]);
+        if (CP && s.comp[VX + c]) v += Cvt<T, C>::ld(res[c][r], s.cinv[VX + c]);
+        return v;
+    };
     constexpr bool RP = DM == 0 || sizeof(T) == 2;
-    C vpre[3][NY];
-    if (RP) load_rmw(k0, vpre);
+    T vpre[3][NY], vpres[3][NY];
+    if (RP) load_rmw(k0, vpre, vpres);
 
     constexpr bool PML = DM != 0, DX = (DM & 1) != 0, DY = (DM & 2) != 0, DZ = (DM & 4) != 0;
     const bool px = DX && in_pml(t.i, g.nx, g.pml);
@@ -202,31 +211,31 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
-            L.sxx[lr][tx + 5] = c_xx[r];
-            L.sxy[lr + 5][tx + 5] = c_xy[r];
-            L.syy[lr + 5][tx] = c_yy[r];
+            L.sxx[lr][tx + 5] = G::use(c_xx[r], t.rin[r], fi_xx);
+            L.sxy[lr + 5][tx + 5] = G::use(c_xy[r], t.rin[r], fi_xy);
+            L.syy[lr + 5][tx] = G::use(c_yy[r], t.rin[r], fi_yy);
             L.b[lr][tx] = qb[r][0];
         }
         if (tx < 10) {
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 const int lr = ty * NY + r;
-                L.sxx[lr][t.hc] = h_xx[r];
-                L.sxy[lr + 5][t.hc] = h_xy[r];
+                L.sxx[lr][t.hc] = G::use(h_xx[r], t.hok[r], fi_xx);
+                L.sxy[lr + 5][t.hc] = G::use(h_xy[r], t.hok[r], fi_xy);
             }
         }
         if (tx == 10) {
 #pragma unroll
-            for (int r = 0; r < NY; ++r) L.b[ty * NY + r][TX] = b_x[r];
+            for (int r = 0; r < NY; ++r) L.b[ty * NY + r][TX] = G::use(b_x[r], t.bxok && t.rin[r], ib);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
             if (ty + BY * m < 10) {
-                L.syy[t.ylr[m]][tx] = y_yy[m];
-                L.sxy[t.ylr[m]][tx + 5] = y_xy[m];
+                L.syy[t.ylr[m]][tx] = G::use(y_yy[m], t.yok[m], fi_yy);
+                L.sxy[t.ylr[m]][tx + 5] = G::use(y_xy[m], t.yok[m], fi_xy);
             }
         }
-        if (ty == 0) L.b[TY][tx] = b_y;
+        if (ty == 0) L.b[TY][tx] = G::use(b_y, t.byok, ib);
         if (lead) stage_lead(10, hn);
         __syncthreads();
 
@@ -239,9 +248,14 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
 #pragma unroll
             for (int r = 0; r < NY; ++r)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) vold[c][r] = vpre[c][r];
+                for (int c = 0; c < 3; ++c) vold[c][r] = rmw_value(vpre, vpres, c, r);
         } else {
-            load_rmw(k, vold);
+            T vr[3][NY], vrs[3][NY];
+            load_rmw(k, vr, vrs);
+#pragma unroll
+            for (int r = 0; r < NY; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) vold[c][r] = rmw_value(vr, vrs, c, r);
         }
         bool pz = false, any[NY];
         C A2 = (C)1, B2 = (C)1, so[NY][3][3], vsr[NY];
@@ -278,25 +292,30 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
             }
         }
 
-        // ---- loads for later: plane k+1, the lead plane k+LD+1, queue heads
-        if (k + 1 < k1) {
-            fetch(k + 1);
-            if (RP) load_rmw(k + 1, vpre);
-        }
-        if (k + LD + 1 < k1) fetch_lead(k + LD + 1, hn);
-        C nxz[NY], nyz[NY], nzz[NY], nb[NY];
+        // ---- loads for later, in the order their values are needed: RMW
+        // operands of plane k+1, queue heads (plane k+6, pushed after this
+        // plane), then the staging values of plane k+1 and of the lead plane
+        // k+LD+1 (staged after the next barrier).  The vector-memory counter
+        // retires in issue order.  Past the end of the box the loads repeat
+        // the box's last plane (cached) instead of being skipped, so no
+        // branch makes the compiler's waits count them conservatively.
+        const int kn = k + 1 < k1 ? k + 1 : k;
+        if (RP) load_rmw(kn, vpre, vpres);
+        S nxz[NY], nyz[NY], nzz[NY], nb[NY];
+        const bool z6 = k + 6 < g.nz, z2 = k + 2 < g.nz;
         {
-            const bool z6 = k + 6 < g.nz, z2 = k + 2 < g.nz;
             const T *pxz = plane(Fxz, z6 ? k + 6 : k, sz), *pyz = plane(Fyz, z6 ? k + 6 : k, sz);
             const T *pzz = plane(Fzz, z6 ? k + 6 : k, sz), *pb = plane(Fb, z2 ? k + 2 : k, sz);
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                nxz[r] = mld<T, C>(pxz, t.ro[r], t.rin[r] && z6, fi_xz);
-                nyz[r] = mld<T, C>(pyz, t.ro[r], t.rin[r] && z6, fi_yz);
-                nzz[r] = mld<T, C>(pzz, t.ro[r], t.rin[r] && z6, fi_zz);
-                nb[r] = mld<T, C>(pb, t.ro[r], t.rin[r] && z2, ib);
+                nxz[r] = G::ld(pxz, t.ro[r], t.rin[r] && z6, fi_xz);
+                nyz[r] = G::ld(pyz, t.ro[r], t.rin[r] && z6, fi_yz);
+                nzz[r] = G::ld(pzz, t.ro[r], t.rin[r] && z6, fi_zz);
+                nb[r] = G::ld(pb, t.ro[r], t.rin[r] && z2, ib);
             }
         }
+        fetch(kn);
+        fetch_lead(k + LD + 1 < k1 ? k + LD + 1 : k1 - 1, hn);
 
         // ---- compute plane k (its delayed derivatives from ring slot sl,
         // which then takes those of the lead plane)
@@ -379,11 +398,11 @@ __device__ __forceinline__ void velocity_tile_ld(const Shot<T, C> &s, int i0, in
             }
 #pragma unroll
             for (int q = 0; q < 9; ++q) qzz[r][q] = qzz[r][q + 1];
-            qxz[r][10] = nxz[r];
-            qyz[r][10] = nyz[r];
-            qzz[r][9] = nzz[r];
+            qxz[r][10] = G::use(nxz[r], t.rin[r] && z6, fi_xz);
+            qyz[r][10] = G::use(nyz[r], t.rin[r] && z6, fi_yz);
+            qzz[r][9] = G::use(nzz[r], t.rin[r] && z6, fi_zz);
             qb[r][0] = qb[r][1];
-            qb[r][1] = nb[r];
+            qb[r][1] = G::use(nb[r], t.rin[r] && z2, ib);
         }
     }
 }
@@ -407,6 +426,8 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
     const long long sz = g.sz;
     TileGeo<TY, NY> t;
     tile_geo<TY, NY, (int)sizeof(T)>(g, i0, i1, j0, j1, t);
+    using G = Stg<T, C>;
+    using S = typename G::S;
 
     const T *Fvx = s.f[VX], *Fvy = s.f[VY], *Fvz = s.f[VZ];
     const T *Fmu = s.mu, *Flam = s.lam;
@@ -432,58 +453,61 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
         for (int r = 0; r < NY; ++r) qvz[r][q] = mld<T, C>(pz_, t.ro[r], t.rin[r], fi_vz);
     }
 
-    // mu staging registers and the 1/mu ring of planes k and k+1
-    C m_c[NY], m_x[NY], m_y, m_xy;
+    // mu staging registers and the 1/mu ring of planes k and k+1; mc: mu
+    // of the centre rows
+    S m_c[NY], m_x[NY], m_y, m_xy;
     auto fetch_mu = [&](int k) {
         const T *pm = plane(Fmu, k, sz);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            m_c[r] = mld<T, C>(pm, t.ro[r], t.rin[r], im);
-            m_x[r] = mld<T, C>(pm, t.bxo[r], t.bxok && t.rin[r], im);
+            m_c[r] = G::ld(pm, t.ro[r], t.rin[r], im);
+            m_x[r] = G::ld(pm, t.bxo[r], t.bxok && t.rin[r], im);
         }
-        m_y = mld<T, C>(pm, t.byo, t.byok, im);
-        m_xy = mld<T, C>(pm, t.bxyo, t.bxyok, im);
+        m_y = G::ld(pm, t.byo, t.byok, im);
+        m_xy = G::ld(pm, t.bxyo, t.bxyok, im);
     };
-    auto stage_mu = [&](int k) {
+    auto stage_mu = [&](int k, C (&mc)[NY]) {
         const int sl = k & 1;
 #pragma unroll
-        for (int r = 0; r < NY; ++r) L.rm[sl][ty * NY + r][tx] = t.rin[r] ? (C)1 / m_c[r] : (C)0;
+        for (int r = 0; r < NY; ++r) {
+            mc[r] = G::use(m_c[r], t.rin[r], im);
+            L.rm[sl][ty * NY + r][tx] = t.rin[r] ? (C)1 / mc[r] : (C)0;
+        }
         if (tx == 10) {
 #pragma unroll
-            for (int r = 0; r < NY; ++r) L.rm[sl][ty * NY + r][TX] = (t.bxok && t.rin[r]) ? (C)1 / m_x[r] : (C)0;
+            for (int r = 0; r < NY; ++r) {
+                const bool ok = t.bxok && t.rin[r];
+                L.rm[sl][ty * NY + r][TX] = ok ? (C)1 / G::use(m_x[r], ok, im) : (C)0;
+            }
         }
         if (ty == 0) {
-            L.rm[sl][TY][tx] = t.byok ? (C)1 / m_y : (C)0;
-            if (tx == 10) L.rm[sl][TY][TX] = t.bxyok ? (C)1 / m_xy : (C)0;
+            L.rm[sl][TY][tx] = t.byok ? (C)1 / G::use(m_y, t.byok, im) : (C)0;
+            if (tx == 10) L.rm[sl][TY][TX] = t.bxyok ? (C)1 / G::use(m_xy, t.bxyok, im) : (C)0;
         }
     };
     C muk[NY], mun[NY];
     fetch_mu(k0);
-#pragma unroll
-    for (int r = 0; r < NY; ++r) muk[r] = m_c[r];
-    stage_mu(k0);
+    stage_mu(k0, muk);
     fetch_mu(k0 + 1);
-#pragma unroll
-    for (int r = 0; r < NY; ++r) mun[r] = m_c[r];
-    stage_mu(k0 + 1);
+    stage_mu(k0 + 1, mun);
 
     // lead-plane staging: x and y halos of the three velocities
     struct Halo {
-        C hx[NY], hy[NY], hz[NY], yx[NH], yy[NH], yz[NH];
+        S hx[NY], hy[NY], hz[NY], yx[NH], yy[NH], yz[NH];
     };
     auto fetch = [&](int k, Halo &h) {
         const T *pvx = plane(Fvx, k, sz), *pvy = plane(Fvy, k, sz), *pvz = plane(Fvz, k, sz);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            h.hx[r] = mld<T, C>(pvx, t.hro[r], t.hok[r], fi_vx);
-            h.hy[r] = mld<T, C>(pvy, t.hro[r], t.hok[r], fi_vy);
-            h.hz[r] = mld<T, C>(pvz, t.hro[r], t.hok[r], fi_vz);
+            h.hx[r] = G::ld(pvx, t.hro[r], t.hok[r], fi_vx);
+            h.hy[r] = G::ld(pvy, t.hro[r], t.hok[r], fi_vy);
+            h.hz[r] = G::ld(pvz, t.hro[r], t.hok[r], fi_vz);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            h.yx[m] = mld<T, C>(pvx, t.yro[m], t.yok[m], fi_vx);
-            h.yy[m] = mld<T, C>(pvy, t.yro[m], t.yok[m], fi_vy);
-            h.yz[m] = mld<T, C>(pvz, t.yro[m], t.yok[m], fi_vz);
+            h.yx[m] = G::ld(pvx, t.yro[m], t.yok[m], fi_vx);
+            h.yy[m] = G::ld(pvy, t.yro[m], t.yok[m], fi_vy);
+            h.yz[m] = G::ld(pvz, t.yro[m], t.yok[m], fi_vz);
         }
     };
     // centres from the queues: vx, vy at index qa, vz at index qa + 1
@@ -499,17 +523,17 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 const int lr = ty * NY + r;
-                L.vx[lr + 5][t.hc] = h.hx[r];
-                L.vy[lr + 5][t.hc] = h.hy[r];
-                L.vz[lr + 5][t.hc] = h.hz[r];
+                L.vx[lr + 5][t.hc] = G::use(h.hx[r], t.hok[r], fi_vx);
+                L.vy[lr + 5][t.hc] = G::use(h.hy[r], t.hok[r], fi_vy);
+                L.vz[lr + 5][t.hc] = G::use(h.hz[r], t.hok[r], fi_vz);
             }
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
             if (ty + BY * m < 10) {
-                L.vx[t.ylr[m]][tx + 5] = h.yx[m];
-                L.vy[t.ylr[m]][tx + 5] = h.yy[m];
-                L.vz[t.ylr[m]][tx + 5] = h.yz[m];
+                L.vx[t.ylr[m]][tx + 5] = G::use(h.yx[m], t.yok[m], fi_vx);
+                L.vy[t.ylr[m]][tx + 5] = G::use(h.yy[m], t.yok[m], fi_vy);
+                L.vz[t.ylr[m]][tx + 5] = G::use(h.yz[m], t.yok[m], fi_vz);
             }
         }
     };
@@ -550,15 +574,17 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
         }
     }
 
-    auto load_rmw = [&](int kk, C (&out)[6][NY], C (&lo)[NY]) {
+    // old stresses and lambda (stored values and residuals), converted where
+    // used
+    auto load_rmw = [&](int kk, T (&out)[6][NY], T (&res)[6][NY], T (&lo)[NY]) {
         const T *pl = plane(Flam, kk, sz);
 #pragma unroll
-        for (int r = 0; r < NY; ++r) lo[r] = Cvt<T, C>::ld(bld_nt(pl, t.ro[r]), il);
+        for (int r = 0; r < NY; ++r) lo[r] = bld_nt(pl, t.ro[r]);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             const T *p = plane(s.f[SXX + c], kk, sz);
 #pragma unroll
-            for (int r = 0; r < NY; ++r) out[c][r] = Cvt<T, C>::ld(bld_nt(p, t.ro[r]), s.finv[SXX + c]);
+            for (int r = 0; r < NY; ++r) out[c][r] = bld_nt(p, t.ro[r]);
         }
         if (CP) {
 #pragma unroll
@@ -566,13 +592,18 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
                 if (s.comp[SXX + c]) {
                     const T *pc = plane(s.comp[SXX + c], kk, sz);
 #pragma unroll
-                    for (int r = 0; r < NY; ++r) out[c][r] += Cvt<T, C>::ld(bld_nt(pc, t.ro[r]), s.cinv[SXX + c]);
+                    for (int r = 0; r < NY; ++r) res[c][r] = bld_nt(pc, t.ro[r]);
                 }
         }
     };
+    auto rmw_value = [&](const T (&raw)[6][NY], const T (&res)[6][NY], int c, int r) -> C {
+        C v = Cvt<T, C>::ld(raw[c][r], s.finv[SXX + c]);
+        if (CP && s.comp[SXX + c]) v += Cvt<T, C>::ld(res[c][r], s.cinv[SXX + c]);
+        return v;
+    };
     constexpr bool RP = DM == 0 || sizeof(T) == 2;
-    C spre[6][NY], lpre[NY];
-    if (RP) load_rmw(k0, spre, lpre);
+    T spre[6][NY], spres[6][NY], lpre[NY];
+    if (RP) load_rmw(k0, spre, spres, lpre);
 
     constexpr bool PML = DM != 0, DX = (DM & 1) != 0, DY = (DM & 2) != 0, DZ = (DM & 4) != 0;
     const bool px = DX && in_pml(t.i, g.nx, g.pml);
@@ -606,12 +637,19 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
         if (RP) {
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                lam[r] = lpre[r];
+                lam[r] = Cvt<T, C>::ld(lpre[r], il);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) old[c][r] = spre[c][r];
+                for (int c = 0; c < 6; ++c) old[c][r] = rmw_value(spre, spres, c, r);
             }
         } else {
-            load_rmw(k, old, lam);
+            T sr[6][NY], srs[6][NY], lr_[NY];
+            load_rmw(k, sr, srs, lr_);
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                lam[r] = Cvt<T, C>::ld(lr_[r], il);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) old[c][r] = rmw_value(sr, srs, c, r);
+            }
         }
         if (k == src_k && s.src_saved) {
 #pragma unroll
@@ -671,26 +709,30 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
             }
         }
 
-        // ---- loads for later: the next plane's operands, the next lead
-        // plane's halos, queue heads (plane k+6)
+        // ---- loads for later, in the order their values are needed: RMW
+        // operands of plane k+1, queue heads (plane k+6) and mu of plane k+2
+        // (used after this plane), the next lead plane's halos (staged after
+        // the next barrier).  The vector-memory counter retires in issue
+        // order.  Past the end of the box the loads repeat planes the box
+        // has read (cached) instead of being skipped, so no branch makes the
+        // compiler's waits count them conservatively.
         const bool more = k + 1 < k1;
-        if (more) {
-            if (RP) load_rmw(k + 1, spre, lpre);
-            fetch_mu(k + 2);
-        }
-        if (k + LD + 1 < k1) fetch(k + LD + 1, hn);
-        C nvx[NY], nvy[NY], nvz[NY];
+        const int kn = more ? k + 1 : k;
+        if (RP) load_rmw(kn, spre, spres, lpre);
+        S nvx[NY], nvy[NY], nvz[NY];
+        const bool z6 = k + 6 < g.nz;
         {
-            const bool z6 = k + 6 < g.nz;
             const T *px_ = plane(Fvx, z6 ? k + 6 : k, sz), *py_ = plane(Fvy, z6 ? k + 6 : k, sz);
             const T *pz_ = plane(Fvz, z6 ? k + 6 : k, sz);
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                nvx[r] = mld<T, C>(px_, t.ro[r], t.rin[r] && z6, fi_vx);
-                nvy[r] = mld<T, C>(py_, t.ro[r], t.rin[r] && z6, fi_vy);
-                nvz[r] = mld<T, C>(pz_, t.ro[r], t.rin[r] && z6, fi_vz);
+                nvx[r] = G::ld(px_, t.ro[r], t.rin[r] && z6, fi_vx);
+                nvy[r] = G::ld(py_, t.ro[r], t.rin[r] && z6, fi_vy);
+                nvz[r] = G::ld(pz_, t.ro[r], t.rin[r] && z6, fi_vz);
             }
         }
+        fetch_mu(kn + 1);
+        fetch(k + LD + 1 < k1 ? k + LD + 1 : k1 - 1, hn);
 
         // the delayed derivatives of plane k from ring slot sl, which then
         // takes those of the lead plane
@@ -808,11 +850,12 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
 
         // ---- ring, queues and delay line advance
         if (more) {
-            stage_mu(k + 2);
+            C m2[NY];
+            stage_mu(k + 2, m2);
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
                 muk[r] = mun[r];
-                mun[r] = m_c[r];
+                mun[r] = m2[r];
             }
         }
 #pragma unroll
@@ -824,9 +867,9 @@ __device__ __forceinline__ void stress_tile_ld(const Shot<T, C> &s, int i0, int 
             }
 #pragma unroll
             for (int q = 0; q < 10; ++q) qvz[r][q] = qvz[r][q + 1];
-            qvx[r][9] = nvx[r];
-            qvy[r][9] = nvy[r];
-            qvz[r][10] = nvz[r];
+            qvx[r][9] = G::use(nvx[r], t.rin[r] && z6, fi_vx);
+            qvy[r][9] = G::use(nvy[r], t.rin[r] && z6, fi_vy);
+            qvz[r][10] = G::use(nvz[r], t.rin[r] && z6, fi_vz);
         }
     }
 }

@@ -268,7 +268,11 @@ class Solver:
     """One shot on the current GPU.  Arrays are numpy, x-fastest, shape (nz, ny, nx)."""
 
     def __init__(self, nx, ny, nz, pml, dt, dx, dy, dz, coef=None, time_length=0, precision="fp32",
-                 source_mode="transient", kernel="tiled"):
+                 source_mode="transient", kernel=None):
+        # kernel None: the tiled sweeps, or for FP64 (the checking build,
+        # which the library runs on the per-point kernels only) "simple"
+        if kernel is None:
+            kernel = "simple" if precision == "fp64" else "tiled"
         c = _Config()
         c.nx, c.ny, c.nz, c.pml = nx, ny, nz, pml
         c.dt, c.dx, c.dy, c.dz = dt, dx, dy, dz
@@ -581,7 +585,7 @@ class ElasticWaveEquation:
 
     def __init__(self, timeLength, gpuDim, pml, grid, delta, medCPU, sourceLocation, stationLocation, dc,
                  mainFrequncy, components=("sxx",), source_mode="accumulate", precision="fp32",
-                 kernel="tiled", damping=None):
+                 kernel=None, damping=None):
         self.timeLength = timeLength
         self.pml = pml
         self.grid = GRID(*grid)
@@ -684,7 +688,7 @@ class ElasticWaveEquation3D:
     the same text as include/ElasticWaveEquation3D.hpp writes."""
 
     def __init__(self, grid, timeLength, pml, buoyancy, lambda_, mu, sourceLocation, deltaX=5.0, deltaY=5.0,
-                 deltaZ=5.0, deltaT=0.0005, mainFrequncy=30.0, precision="fp32", kernel="tiled"):
+                 deltaZ=5.0, deltaT=0.0005, mainFrequncy=30.0, precision="fp32", kernel=None):
         self.grid = GRID(*grid)
         self.timeLength = int(timeLength)
         self.pml = int(pml)

@@ -79,7 +79,7 @@ class Workload:
             s.close()
         return scales_for_maxabs(m)
 
-    def solver(self, precision="fp32", kernel="tiled", source_mode="transient", time_length=None, scales=None):
+    def solver(self, precision="fp32", kernel=None, source_mode="transient", time_length=None, scales=None):
         """A solver for this workload.  FP16/BF16 storage scales come from
         `scales` or, if None, from an FP32 pilot run (cached)."""
         if precision in ("fp16", "bf16") and scales is None:

@@ -48,6 +48,15 @@ def test_create_rejects_bad_configurations(L, kw, needle):
     assert needle in L.sg_last_error()
 
 
+def test_fp64_runs_the_per_point_kernels_only(L):
+    # FP64 (precision 1) with the tiled sweeps (kernel 0) is refused before
+    # any device call; FP64 solvers use SG_KERNEL_SIMPLE
+    h = ctypes.c_void_p(1)
+    assert L.sg_create(ctypes.byref(_cfg(precision=1, kernel=0)), ctypes.byref(h)) == -6  # SG_ERR_UNSUPPORTED
+    assert h.value is None
+    assert b"FP64" in L.sg_last_error()
+
+
 def test_tiled_split_arrays_limited_to_4gib_each(L):
     # 16384x16384 plane, P8: each split array holds 2*8*2^28 z-slab slots,
     # 16 GiB in FP32, beyond the tiled sweeps' 32-bit offsets
