@@ -11,9 +11,25 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "staggered-grid_amd"))
 
 
+_CONFIG = None
+
+
 def pytest_configure(config):
+    global _CONFIG
+    _CONFIG = config
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
     config.addinivalue_line("markers", "slow: CPU test that takes more than ~20 s")
+
+
+def say(msg):
+    """A progress line printed while a test runs (past pytest's output
+    capture), so a long GPU test shows that it is working."""
+    capman = _CONFIG.pluginmanager.getplugin("capturemanager") if _CONFIG is not None else None
+    if capman is None:
+        print(f"[gpu] {msg}", flush=True)
+        return
+    with capman.global_and_fixture_disabled():
+        print(f"[gpu] {msg}", flush=True)
 
 
 def load_golden(name):

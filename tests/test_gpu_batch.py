@@ -3,14 +3,17 @@
 The per-point arithmetic does not depend on how points are grouped into
 tiles, z-chunks or batches, so all of these must be bit-identical to a single
 solver with the default tile."""
+import sys
 import numpy as np
 import pytest
+
+from conftest import say
 
 pytestmark = pytest.mark.gpu
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 @pytest.fixture(scope="module")

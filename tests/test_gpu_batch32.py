@@ -12,14 +12,17 @@ shot replaces one run() of the reference CUDA driver
   of the same kernels, rel-L2 <= 1e-5 on the receivers; the final fields,
   which hold the FP32 rounding of 800 steps (1.05e-5 measured), <= 3e-5.
 """
+import sys
 import numpy as np
 import pytest
+
+from conftest import say
 
 pytestmark = pytest.mark.gpu
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 def _rel(a, b):

@@ -17,13 +17,14 @@ and the ported main program, staggered-grid_amd/examples/main_cpu.cpp.
   Python mirror of the class (sgfd.ElasticWaveEquation3D) must write the
   same files byte for byte.
 """
+import sys
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import load_golden
+from conftest import load_golden, say
 
 pytestmark = pytest.mark.gpu
 
@@ -31,7 +32,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 def _rel(a, b):

@@ -32,14 +32,17 @@
   1e-17, only 21 decades above the floor, so the differences reach the signal
   (rel-L2 ~1e-8) and FP32 is held to rel-L2 <= 1e-6.
 """
+import sys
 import numpy as np
 import pytest
+
+from conftest import say
 
 pytestmark = pytest.mark.gpu
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 def _rel(a, b):

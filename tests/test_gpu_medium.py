@@ -8,8 +8,11 @@ to rel-L2 <= 1e-5; V = v0 (1 + delta) then agrees to ~1e-7 relative.  At
 through properties that do not depend on the FFT's rounding: zero mean and
 standard deviation eps, and seed determinism.
 """
+import sys
 import numpy as np
 import pytest
+
+from conftest import say
 
 import medium as M
 
@@ -17,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 def _rel(a, b):

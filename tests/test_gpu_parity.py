@@ -4,12 +4,13 @@ Tolerance: relative L2 <= 1e-5 for FP32 wavefields and station traces after
 the full run (the stated parity bar); FP64 against the FP64 oracle <= 1e-11.
 Both kernel variants (tiled, simple) are checked.
 """
+import sys
 import os
 
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_golden, random_case as _random_case
+from conftest import GOLDEN, load_golden, random_case as _random_case, say
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +25,7 @@ def _rel(a, b):
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 @pytest.fixture(scope="module")

@@ -5,14 +5,17 @@ There is no 16-bit counterpart in the reference (its solvers are FP32), so
 these results are checked against the FP64 build of the same kernels on the
 same inputs; the tolerances are the measured error levels of each mode with
 margin (plain FP16 rounding noise grows with the number of steps)."""
+import sys
 import numpy as np
 import pytest
+
+from conftest import say
 
 pytestmark = pytest.mark.gpu
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 @pytest.fixture(scope="module")

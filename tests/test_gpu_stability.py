@@ -12,10 +12,11 @@
   compiled reference's fields of G1 (<= 1e-5); sg_energy returns the first
   two values; two calls give the same bits (fixed summation order).
 """
+import sys
 import numpy as np
 import pytest
 
-from conftest import load_golden
+from conftest import load_golden, say
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +24,7 @@ ERR_INVALID, ERR_UNSTABLE = -1, -7
 
 
 def _say(msg):
-    print(f"[gpu] {msg}", flush=True)
+    say(msg)
 
 
 @pytest.fixture(scope="module")
