@@ -298,6 +298,15 @@ __device__ __forceinline__ void gst<bf16>(bf16 *p, unsigned off, bf16 v)
     ((__attribute__((address_space(1))) uint16_t *)p)[off] = v.bits;
 }
 
+// Old PML splits of single-axis face boxes loaded one plane ahead: bit 0
+// velocity sweep, bit 1 stress sweep.  Measured at 512x512x384 (300 steps):
+// velocity sweep FP32 1.529 -> 1.505 ms, FP16 1.216 -> 1.164; stress sweep
+// FP32 1.894 -> 1.910, FP16 1.546 -> 1.556 (five split operands per point
+// held across the plane), so the velocity sweep only.
+#ifndef SG_SPLIT_PREFETCH
+#define SG_SPLIT_PREFETCH 1
+#endif
+
 // masked load at a byte offset that is always inside the array
 template <class T, class C>
 __device__ __forceinline__ C mld(const T *__restrict__ p, unsigned boff, bool ok, float inv)
@@ -599,6 +608,24 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         A1[r] = DY ? gld(s.dA[1], jc) : (C)1;
         B1[r] = DY ? gld(s.dB[1], jc) : (C)1;
     }
+    // boxes that damp along one axis load the old splits of the next plane
+    // with the other operands of the next plane (every lane loads, at slot 0
+    // where its point does not damp; the values are used only where it does)
+    constexpr bool SPF = (SG_SPLIT_PREFETCH & 1) && (DM == 1 || DM == 2 || DM == 4);
+    constexpr int AX = DM == 1 ? 0 : (DM == 2 ? 1 : 2);
+    C spo[NY][3], spv[NY];
+    auto load_splits = [&](int kk) {
+        const bool pzk = DZ && in_pml(kk, g.nz, g.pml);
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            const bool an = (px || py || pzk) && t.rup[r];
+            const unsigned o = an ? (unsigned)(split_slot(g, t.i, t.jr[r], kk) * (long long)sizeof(C)) : 0u;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) spo[r][c] = bld(s.sv[3 * c + AX], o);
+            spv[r] = s.vs ? bld(s.vs, o) : (C)0;
+        }
+    };
+    if (SPF) load_splits(k0);
 
     for (int k = k0; k < k1; ++k) {
         auto &L = vplane(Ls, k);
@@ -694,6 +721,12 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int a2 = 0; a2 < 3; ++a2) so[r][c][a2] = (C)0;
+                if (SPF) {
+                    vsr[r] = spv[r];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) so[r][c][AX] = spo[r][c];
+                    continue;
+                }
                 vsr[r] = (s.vs && any[r]) ? bld(s.vs, sb[r]) : (C)0;
                 if (any[r]) {
                     if (DX && px) {
@@ -723,6 +756,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         // iteration.
         const int kn = k + 1 < k1 ? k + 1 : k;
         if (RP) load_rmw(kn, vpre, vpres);
+        if (SPF) load_splits(kn);
         S nxz[NY], nyz[NY], nzz[NY], nb[NY];
         const bool z5 = k + 5 < g.nz, z6 = k + 6 < g.nz, z2 = k + 2 < g.nz;
         {
@@ -977,6 +1011,28 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     }
     const int src_k = s.src >= 0 ? (int)(s.src / sz) : -1;
     const unsigned src_o = s.src >= 0 ? (unsigned)(s.src - (long long)src_k * sz) * (unsigned)sizeof(T) : 0u;
+    // single-axis face boxes: old splits of the next plane loaded with its
+    // other operands, as in the velocity sweep (the normal components' splits
+    // on the damped axis, then the two shear components that have one)
+    constexpr bool SPF = (SG_SPLIT_PREFETCH & 2) && (DM == 1 || DM == 2 || DM == 4);
+    constexpr int AX = DM == 1 ? 0 : (DM == 2 ? 1 : 2);
+    constexpr int SH0 = DM == 1 ? XY_X : (DM == 2 ? XY_Y : XZ_Z);
+    constexpr int SH1 = DM == 1 ? XZ_X : (DM == 2 ? YZ_Y : YZ_Z);
+    C spo[NY][5], spv[NY];
+    auto load_splits = [&](int kk) {
+        const bool pzk = DZ && in_pml(kk, g.nz, g.pml);
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            const bool an = (px || py || pzk) && t.rup[r];
+            const unsigned o = an ? (unsigned)(split_slot(g, t.i, t.jr[r], kk) * (long long)sizeof(C)) : 0u;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) spo[r][c] = bld(s.ss[XX_X + 3 * c + AX], o);
+            spo[r][3] = bld(s.ss[SH0], o);
+            spo[r][4] = bld(s.ss[SH1], o);
+            spv[r] = s.vs ? bld(s.vs, o) : (C)0;
+        }
+    };
+    if (SPF) load_splits(k0);
 
     for (int k = k0; k < k1; ++k) {
         auto &L = vplane(Ls, k);
@@ -1075,6 +1131,17 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
 #pragma unroll
                     for (int a2 = 0; a2 < 2; ++a2) s2[r][c][a2] = (C)0;
                 }
+                if (SPF) {
+                    // normal splits on axis AX; shear: xy and xz store x first,
+                    // xy and yz y, xz and yz z (the slot of each axis in s2)
+                    vsr[r] = spv[r];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) sn3[r][c][AX] = spo[r][c];
+                    if (DM == 1) { s2[r][0][0] = spo[r][3]; s2[r][1][0] = spo[r][4]; }
+                    if (DM == 2) { s2[r][0][1] = spo[r][3]; s2[r][2][0] = spo[r][4]; }
+                    if (DM == 4) { s2[r][1][1] = spo[r][3]; s2[r][2][1] = spo[r][4]; }
+                    continue;
+                }
                 vsr[r] = (s.vs && any[r]) ? bld(s.vs, sb[r]) : (C)0;
                 if (any[r]) {
                     const unsigned o = sb[r];
@@ -1125,6 +1192,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         fetch_mu(kn + 1);  // staged into the ring after this plane
         fetch(kn);
         if (RP) load_rmw(kn, spre, spres, lpre);
+        if (SPF) load_splits(kn);
 
         auto &R0 = rmslot(Ls, k);
         auto &R1 = rmslot(Ls, k + 1);
