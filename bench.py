@@ -245,6 +245,8 @@ def precision_study(args):
     modes = [("fp64", "fp64", None, None, None), ("fp32", "fp32", None, None, None),
              ("fp16", "fp16", None, None, None), ("bf16", "bf16", None, None, None),
              ("fp16-stochastic", "fp16", "stochastic", None, None), ("fp16-compensated", "fp16", None, "all", None),
+             ("fp16-compensated-velocity", "fp16", None, "velocity", None),
+             ("fp16-compensated-stress", "fp16", None, "stress", None),
              ("bf16-stochastic", "bf16", "stochastic", None, None), ("bf16-compensated", "bf16", None, "all", None),
              ("fp32-model-fp16", "fp32", None, None, "model16"), ("fp16-scales-x1/8", "fp16", None, None, 0.125),
              ("fp16-scales-x4", "fp16", None, None, 4.0)]
