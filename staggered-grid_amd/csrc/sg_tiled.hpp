@@ -307,6 +307,43 @@ __device__ __forceinline__ void gst<bf16>(bf16 *p, unsigned off, bf16 v)
 #define SG_SPLIT_PREFETCH 1
 #endif
 
+// FP32 staging rows loaded straight into LDS (global_load_lds_dword) instead
+// of through registers: the y-halo rows of both sweeps and the velocity
+// sweep's centre rows of sxx, sxy, syy.  A bit per box list (1 interior, 2 x
+// faces, 4 y faces, 8 z faces, 16 edges and corners); those lists use the
+// double-buffered staging, so the rows of plane k+1 land in the other buffer
+// while plane k is read.  Values outside the array are not zeroed, as the
+// register path zeroes them; no updated point reads them (its taps stay
+// inside [0, n)).  Measured on MI355X with the interior lists only
+// (SG_LDSDMA=1): their kernels drop from 137-144 to 113-122 VGPRs (4 waves
+// per SIMD instead of 3), 500^3 without PML 30.28 -> 31.31 Gpt/s, but
+// 512x512x384 P20 and 256^3 P16 unchanged (29.45 / 29.42, 24.26 / 24.27)
+// while the interior lists fetch 8-9 % more (more blocks share each L2);
+// with the PML lists as well it is slower (512x512x384: 28.96 with the x
+// faces, 28.82 with every list).  Off by default; worth turning on for grids
+// without PML.
+#ifndef SG_LDSDMA
+#define SG_LDSDMA 0
+#endif
+constexpr int list_bit(int dm) { return dm == 0 ? 0 : (dm == 1 ? 1 : (dm == 2 ? 2 : (dm == 4 ? 3 : 4))); }
+template <class T, int DM>
+struct DmaOn {
+    static constexpr bool v = std::is_same<T, float>::value && ((SG_LDSDMA >> list_bit(DM)) & 1) != 0;
+};
+
+// lane l of the wave loads the 32-bit value at p + boff into dst[l] (dst is
+// the same for every lane of the wave)
+__device__ __forceinline__ void lds_dma(const float *p, unsigned boff, float *dst)
+{
+    typedef __attribute__((address_space(1))) const char gcc;
+    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) const void *)((gcc *)p + boff),
+                                     (__attribute__((address_space(3))) void *)dst, 4, 0, 0);
+}
+
+// wait until every vector-memory load of the wave has completed (the LDS rows
+// above), before the barrier that publishes them
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // masked load at a byte offset that is always inside the array
 template <class T, class C>
 __device__ __forceinline__ C mld(const T *__restrict__ p, unsigned boff, bool ok, float inv)
@@ -535,26 +572,55 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     using S = typename G::S;
     S c_xx[NY], c_xy[NY], c_yy[NY], h_xx[NY], h_xy[NY], h_xz[NY], b_x[NY];
     S y_yy[NH], y_yz[NH], y_xy[NH], b_y;
+    // centre rows of sxx, sxy, syy and the y-halo rows straight into LDS
+    constexpr bool YD = DmaOn<T, DM>::v && IsDB<LS>::value && !XL;
     auto fetch = [&](int k) {
         const T *pxx = plane(Fxx, k, sz), *pxy = plane(Fxy, k, sz), *pyy = plane(Fyy, k, sz);
         const T *pxz = plane(Fxz, k, sz), *pyz = plane(Fyz, k, sz), *pb = plane(Fb, k, sz);
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            c_xx[r] = G::ld(pxx, t.ro[r], t.rin[r], fi_xx);
-            c_xy[r] = G::ld(pxy, t.ro[r], t.rin[r], fi_xy);
-            c_yy[r] = G::ld(pyy, t.ro[r], t.rin[r], fi_yy);
+            if constexpr (!YD) {
+                c_xx[r] = G::ld(pxx, t.ro[r], t.rin[r], fi_xx);
+                c_xy[r] = G::ld(pxy, t.ro[r], t.rin[r], fi_xy);
+                c_yy[r] = G::ld(pyy, t.ro[r], t.rin[r], fi_yy);
+            }
             h_xx[r] = G::ld(pxx, t.hro[r], t.hok[r], fi_xx);
             h_xy[r] = G::ld(pxy, t.hro[r], t.hok[r], fi_xy);
             h_xz[r] = G::ld(pxz, t.hro[r], t.hok[r], fi_xz);
             b_x[r] = G::ld(pb, t.bxo[r], t.bxok && t.rin[r], ib);
         }
+        if constexpr (YD) {
+            b_y = G::ld(pb, t.byo, t.byok, ib);
+            auto &Ln = vplane(Ls, k);
 #pragma unroll
-        for (int m = 0; m < NH; ++m) {
-            y_yy[m] = G::ld(pyy, t.yro[m], t.yok[m], fi_yy);
-            y_yz[m] = G::ld(pyz, t.yro[m], t.yok[m], fi_yz);
-            y_xy[m] = G::ld(pxy, t.yro[m], t.yok[m], fi_xy);
+            for (int r = 0; r < NY; ++r) {
+                const int lr = ty * NY + r;
+                lds_dma((const float *)pxx, t.ro[r], &Ln.sxx[lr][5]);
+                lds_dma((const float *)pxy, t.ro[r], &Ln.sxy[lr + 5][5]);
+                lds_dma((const float *)pyy, t.ro[r], &Ln.syy[lr + 5][0]);
+            }
+            // a wave with no row in slot m (ty + BY*m >= 10) loads its row of
+            // slot m-1 again, so every wave issues the same loads; with more
+            // than ten waves per block (BY > 10) waves 10.. have no row at all
+#pragma unroll
+            for (int m = 0; m < NH; ++m) {
+                const bool own = ty + BY * m < 10;
+                if (BY > 10 && !own) continue;
+                const int lr = own ? t.ylr[m] : t.ylr[m > 0 ? m - 1 : 0];
+                const unsigned o = own ? t.yro[m] : t.yro[m > 0 ? m - 1 : 0];
+                lds_dma((const float *)pyy, o, &Ln.syy[lr][0]);
+                lds_dma((const float *)pyz, o, &Ln.syz[lr][0]);
+                lds_dma((const float *)pxy, o, &Ln.sxy[lr][5]);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < NH; ++m) {
+                y_yy[m] = G::ld(pyy, t.yro[m], t.yok[m], fi_yy);
+                y_yz[m] = G::ld(pyz, t.yro[m], t.yok[m], fi_yz);
+                y_xy[m] = G::ld(pxy, t.yro[m], t.yok[m], fi_xy);
+            }
+            b_y = G::ld(pb, t.byo, t.byok, ib);
         }
-        b_y = G::ld(pb, t.byo, t.byok, ib);
     };
     fetch(k0);
 
@@ -633,8 +699,10 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         C cxx[NY], cxy[NY], hxx[NY], hxy[NY], hxz[NY];
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            cxx[r] = G::use(c_xx[r], t.rin[r], fi_xx);
-            cxy[r] = G::use(c_xy[r], t.rin[r], fi_xy);
+            if constexpr (!YD) {
+                cxx[r] = G::use(c_xx[r], t.rin[r], fi_xx);
+                cxy[r] = G::use(c_xy[r], t.rin[r], fi_xy);
+            }
             hxx[r] = G::use(h_xx[r], t.hok[r], fi_xx);
             hxy[r] = G::use(h_xy[r], t.hok[r], fi_xy);
             hxz[r] = G::use(h_xz[r], t.hok[r], fi_xz);
@@ -653,9 +721,11 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int lr = ty * NY + r;
-            if (!XL) L.sxx[lr][tx + 5] = cxx[r];
-            L.sxy[lr + 5][tx + 5] = cxy[r];
-            L.syy[lr + 5][tx] = G::use(c_yy[r], t.rin[r], fi_yy);
+            if constexpr (!YD) {
+                if (!XL) L.sxx[lr][tx + 5] = cxx[r];
+                L.sxy[lr + 5][tx + 5] = cxy[r];
+                L.syy[lr + 5][tx] = G::use(c_yy[r], t.rin[r], fi_yy);
+            }
             if (!XL) L.sxz[lr][tx + 5] = qxz[r][5];
             L.syz[lr + 5][tx] = qyz[r][5];
             L.b[lr][tx] = qb[r][0];
@@ -675,13 +745,14 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            if (ty + BY * m < 10) {
+            if (!YD && ty + BY * m < 10) {
                 L.syy[t.ylr[m]][tx] = G::use(y_yy[m], t.yok[m], fi_yy);
                 L.syz[t.ylr[m]][tx] = G::use(y_yz[m], t.yok[m], fi_yz);
                 L.sxy[t.ylr[m]][tx + 5] = G::use(y_xy[m], t.yok[m], fi_xy);
             }
         }
         if (ty == 0) L.b[TY][tx] = G::use(b_y, t.byok, ib);
+        if constexpr (YD) wait_vm_all();
         __syncthreads();
 
         // ---- read half of the read-modify-write of plane k: loaded one plane
@@ -940,6 +1011,8 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     // staging registers: x and y halos of the three velocities (stored
     // values, converted and masked when staged)
     S h_vx[NY], h_vy[NY], h_vz[NY], y_vx[NH], y_vy[NH], y_vz[NH];
+    // y-halo rows straight into LDS
+    constexpr bool YD = DmaOn<T, DM>::v && IsDB<LS>::value;
     auto fetch = [&](int k) {
         const T *pvx = plane(Fvx, k, sz), *pvy = plane(Fvy, k, sz), *pvz = plane(Fvz, k, sz);
 #pragma unroll
@@ -948,11 +1021,28 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
             h_vy[r] = G::ld(pvy, t.hro[r], t.hok[r], fi_vy);
             h_vz[r] = G::ld(pvz, t.hro[r], t.hok[r], fi_vz);
         }
+        if constexpr (YD) {
+            auto &Ln = vplane(Ls, k);
+            // a wave with no row in slot m (ty + BY*m >= 10) loads its row of
+            // slot m-1 again, so every wave issues the same loads; with more
+            // than ten waves per block (BY > 10) waves 10.. have no row at all
 #pragma unroll
-        for (int m = 0; m < NH; ++m) {
-            y_vx[m] = G::ld(pvx, t.yro[m], t.yok[m], fi_vx);
-            y_vy[m] = G::ld(pvy, t.yro[m], t.yok[m], fi_vy);
-            y_vz[m] = G::ld(pvz, t.yro[m], t.yok[m], fi_vz);
+            for (int m = 0; m < NH; ++m) {
+                const bool own = ty + BY * m < 10;
+                if (BY > 10 && !own) continue;
+                const int lr = own ? t.ylr[m] : t.ylr[m > 0 ? m - 1 : 0];
+                const unsigned o = own ? t.yro[m] : t.yro[m > 0 ? m - 1 : 0];
+                lds_dma((const float *)pvx, o, &Ln.vx[lr][5]);
+                lds_dma((const float *)pvy, o, &Ln.vy[lr][5]);
+                lds_dma((const float *)pvz, o, &Ln.vz[lr][5]);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < NH; ++m) {
+                y_vx[m] = G::ld(pvx, t.yro[m], t.yok[m], fi_vx);
+                y_vy[m] = G::ld(pvy, t.yro[m], t.yok[m], fi_vy);
+                y_vz[m] = G::ld(pvz, t.yro[m], t.yok[m], fi_vz);
+            }
         }
     };
     fetch(k0);
@@ -1064,12 +1154,13 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            if (ty + BY * m < 10) {
+            if (!YD && ty + BY * m < 10) {
                 L.vx[t.ylr[m]][tx + 5] = G::use(y_vx[m], t.yok[m], fi_vx);
                 L.vy[t.ylr[m]][tx + 5] = G::use(y_vy[m], t.yok[m], fi_vy);
                 L.vz[t.ylr[m]][tx + 5] = G::use(y_vz[m], t.yok[m], fi_vz);
             }
         }
+        if constexpr (YD) wait_vm_all();
         __syncthreads();
 
         // ---- read half of the read-modify-write of plane k: loaded one plane
@@ -1367,13 +1458,15 @@ struct LeadOn {
 // the LDS of a velocity / stress box for the chosen form
 template <class T, class C, int TY, int NY, int DM>
 struct VelLdsOf {
+    static constexpr bool db = SG_DB != 0 || DmaOn<T, DM>::v;
     using type = typename std::conditional<LeadOn<T, DM, 0, TY, NY>::v, VelLdsL<C, TY>,
-                                           typename std::conditional<SG_DB != 0, VelLdsDB<C, TY>, VelLds<C, TY>>::type>::type;
+                                           typename std::conditional<db, VelLdsDB<C, TY>, VelLds<C, TY>>::type>::type;
 };
 template <class T, class C, int TY, int NY, int DM>
 struct StrLdsOf {
+    static constexpr bool db = SG_DB != 0 || DmaOn<T, DM>::v;
     using type = typename std::conditional<LeadOn<T, DM, 1, TY, NY>::v, StrLdsL<C, TY, DM == 0 ? 5 : 6>,
-                                           typename std::conditional<SG_DB != 0, StrLdsDB<C, TY>, StrLds<C, TY>>::type>::type;
+                                           typename std::conditional<db, StrLdsDB<C, TY>, StrLds<C, TY>>::type>::type;
 };
 
 template <class T, class C, int TY, int NY, int DM, bool CP, class LT>
@@ -1540,12 +1633,15 @@ struct FusedLds {
     static constexpr int mask = sizeof(T) == 8 ? 0
                                 : sizeof(T) == 2 ? (W == 0 ? SG_LEAD_V16 : SG_LEAD_S16)
                                                  : (W == 0 ? SG_LEAD_V32 : SG_LEAD_S32);
+    // (the fused launch keeps single-buffered staging, so its interior boxes
+    // stage through registers: double-buffered, 100^3 P10 ran 0.5 % slower)
+    static constexpr bool db = SG_DB != 0;
     using type = typename std::conditional<
         W == 0,
         typename std::conditional<(mask & 0x1F) != 0, VelLdsL<C, 4>,
-                                  typename std::conditional<SG_DB != 0, VelLdsDB<C, 4>, VelLds<C, 4>>::type>::type,
+                                  typename std::conditional<db, VelLdsDB<C, 4>, VelLds<C, 4>>::type>::type,
         typename std::conditional<(mask & 0x1F) != 0, StrLdsL<C, 4, 6>,
-                                  typename std::conditional<SG_DB != 0, StrLdsDB<C, 4>, StrLds<C, 4>>::type>::type>::type;
+                                  typename std::conditional<db, StrLdsDB<C, 4>, StrLds<C, 4>>::type>::type>::type;
 };
 
 // WV: minimum waves per SIMD the register allocation must allow (0: no
