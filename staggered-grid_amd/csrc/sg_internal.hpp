@@ -167,8 +167,10 @@ hipError_t launch_velocity_simple(int prec, const void *shot, hipStream_t st);
 hipError_t launch_stress_simple(int prec, const void *shot, hipStream_t st);
 // tiled sweeps over a work-list: which 0 = velocity, 1 = stress; shots and
 // items in device memory; dm = damping mask of the list's boxes (0 interior,
-// 1/2/4 x/y/z PML face, 7 edges and corners); comp != 0 selects the
-// compensated 16-bit instantiations; grid 0 = one block per item
+// 1/2/4 x/y/z PML face, 7 edges and corners, DM_INTERIOR_LDS interior boxes
+// whose staging rows are loaded straight into LDS, FP32 only); comp != 0
+// selects the compensated 16-bit instantiations; grid 0 = one block per item
+constexpr int DM_INTERIOR_LDS = 8;
 hipError_t launch_sweep(int which, int prec, int ty, int ny, int dm, int comp, const void *shots_dev,
                         const WorkItem *items, int nitems, int grid, hipStream_t st);
 hipError_t sweep_fp32(int which, int ty, int ny, int dm, int comp, const void *shots, const WorkItem *items, int nitems,

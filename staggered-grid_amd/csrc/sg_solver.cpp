@@ -338,13 +338,18 @@ int enqueue_step(sg_solver *s, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
                 SG_HIP(hipEventRecord(s->fork, s->stream));
                 for (int q = 0; q < ns - 1; ++q) SG_HIP(hipStreamWaitEvent(s->side[q], s->fork, 0));
             }
+            // without PML boxes the interior list runs alone and its FP32
+            // kernels load their staging rows straight into LDS (higher
+            // occupancy; beside the PML lists that gains nothing)
+            const bool lds_rows = s->prec == SG_FP32 && s->comp_mask == 0 && s->wl_off[1] == s->wl_off[NLIST];
             for (int l = 0; l < NLIST; ++l) {
                 // list -> stream: 1 stream: all on the solver stream; 2: PML lists on side 0;
                 // 3: x faces on side 0, the other PML lists on side 1; 4: edges on side 2
                 static const int map[5][NLIST] = {{0}, {0, 0, 0, 0, 0}, {0, 1, 1, 1, 1}, {0, 1, 2, 2, 2}, {0, 1, 2, 2, 3}};
                 const int q = map[ns][l];
                 hipStream_t st = q == 0 ? s->stream : s->side[q - 1];
-                SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, LIST_DM[l], s->comp_mask != 0,
+                const int dm = (l == 0 && lds_rows) ? DM_INTERIOR_LDS : LIST_DM[l];
+                SG_HIP(launch_sweep(which, s->prec, l ? t.pty : t.ty, l ? t.pny : t.ny, dm, s->comp_mask != 0,
                                     s->d_shot, s->d_wl + s->wl_off[l], s->wl_off[l + 1] - s->wl_off[l], 0, st));
             }
             if (ns > 1) {

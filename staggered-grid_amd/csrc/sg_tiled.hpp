@@ -99,12 +99,28 @@ template <class C, int TY>
 __device__ __forceinline__ C (&rmslot(StrLds<C, TY> &L, int k))[TY + 1][TX + 1] { return L.rm[k & 1]; }
 template <class C, int TY>
 __device__ __forceinline__ C (&rmslot(StrLdsDB<C, TY> &L, int k))[TY + 1][TX + 1] { return L.rm[k % 3]; }
+// the double-buffered staging of the interior kernels that load their
+// staging rows straight into LDS (chosen at run time for grids without PML)
+template <class C, int TY>
+struct VelLdsDMA : VelLdsDB<C, TY> {};
+template <class C, int TY>
+struct StrLdsDMA : StrLdsDB<C, TY> {};
 template <class LS>
 struct IsDB : std::false_type {};
 template <class C, int TY>
 struct IsDB<VelLdsDB<C, TY>> : std::true_type {};
 template <class C, int TY>
 struct IsDB<StrLdsDB<C, TY>> : std::true_type {};
+template <class C, int TY>
+struct IsDB<VelLdsDMA<C, TY>> : std::true_type {};
+template <class C, int TY>
+struct IsDB<StrLdsDMA<C, TY>> : std::true_type {};
+template <class LS>
+struct IsDMA : std::false_type {};
+template <class C, int TY>
+struct IsDMA<VelLdsDMA<C, TY>> : std::true_type {};
+template <class C, int TY>
+struct IsDMA<StrLdsDMA<C, TY>> : std::true_type {};
 
 // 5-pair staggered sums: forward at c uses f[c+n] - f[c+1-n], backward uses
 // f[c+n-1] - f[c-n], n = 1..5, with stride st.
@@ -573,7 +589,7 @@ __device__ __forceinline__ void velocity_tile(const Shot<T, C> &s, int i0, int i
     S c_xx[NY], c_xy[NY], c_yy[NY], h_xx[NY], h_xy[NY], h_xz[NY], b_x[NY];
     S y_yy[NH], y_yz[NH], y_xy[NH], b_y;
     // centre rows of sxx, sxy, syy and the y-halo rows straight into LDS
-    constexpr bool YD = DmaOn<T, DM>::v && IsDB<LS>::value && !XL;
+    constexpr bool YD = (DmaOn<T, DM>::v || IsDMA<LS>::value) && std::is_same<T, float>::value && IsDB<LS>::value && !XL;
     auto fetch = [&](int k) {
         const T *pxx = plane(Fxx, k, sz), *pxy = plane(Fxy, k, sz), *pyy = plane(Fyy, k, sz);
         const T *pxz = plane(Fxz, k, sz), *pyz = plane(Fyz, k, sz), *pb = plane(Fb, k, sz);
@@ -1012,7 +1028,7 @@ __device__ __forceinline__ void stress_tile(const Shot<T, C> &s, int i0, int i1,
     // values, converted and masked when staged)
     S h_vx[NY], h_vy[NY], h_vz[NY], y_vx[NH], y_vy[NH], y_vz[NH];
     // y-halo rows straight into LDS
-    constexpr bool YD = DmaOn<T, DM>::v && IsDB<LS>::value;
+    constexpr bool YD = (DmaOn<T, DM>::v || IsDMA<LS>::value) && std::is_same<T, float>::value && IsDB<LS>::value;
     auto fetch = [&](int k) {
         const T *pvx = plane(Fvx, k, sz), *pvy = plane(Fvy, k, sz), *pvz = plane(Fvz, k, sz);
 #pragma unroll
@@ -1566,11 +1582,50 @@ static hipError_t sweep_tnd(int which, const void *shots, const WorkItem *items,
     return hipGetLastError();
 }
 
+// Interior boxes with the staging rows loaded straight into LDS (FP32): the
+// same arithmetic as k_velocity_wl / k_stress_wl with damping mask 0.  The
+// solver launches them for grids without PML, where the interior list runs
+// alone (measured on MI355X, 500^3: 30.28 -> 31.31 Gpt/s; beside the PML
+// lists they gain nothing).
+template <class T, class C, int TY, int NY>
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, 0, NY, 0, false>::v))) void k_velocity_wl_lds(
+    const Shot<T, C> *__restrict__ shots, const WorkItem *__restrict__ items, int nitems)
+{
+    __shared__ VelLdsDMA<C, TY> L;
+    for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
+        const WorkItem w = items[p];
+        if (w.shot >= 0) velocity_tile<T, C, TY, NY, 0, false>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        __syncthreads();
+    }
+}
+
+template <class T, class C, int TY, int NY>
+__global__ __launch_bounds__(TX *TY / NY) __attribute__((amdgpu_waves_per_eu(WavesCap<T, 0, NY, 1, false>::v))) void k_stress_wl_lds(
+    const Shot<T, C> *__restrict__ shots, const WorkItem *__restrict__ items, int nitems)
+{
+    __shared__ StrLdsDMA<C, TY> L;
+    for (int p = blockIdx.x; p < nitems; p += gridDim.x) {
+        const WorkItem w = items[p];
+        if (w.shot >= 0) stress_tile<T, C, TY, NY, 0, false>(shots[w.shot], w.i0, w.i1, w.j0, w.j1, w.k0, w.k1, L);
+        __syncthreads();
+    }
+}
+
 template <class T, class C, int TY, int NY, bool CP>
 static hipError_t sweep_tnc(int which, int dm, const void *shots, const WorkItem *items, int nitems, int grid,
                             hipStream_t st)
 {
     switch (dm) {
+    case DM_INTERIOR_LDS:
+        if constexpr (std::is_same<T, float>::value && !CP) {
+            const Shot<T, C> *sh = static_cast<const Shot<T, C> *>(shots);
+            const dim3 block(TX, TY / NY), g(grid);
+            if (which == 0) hipLaunchKernelGGL((k_velocity_wl_lds<T, C, TY, NY>), g, block, 0, st, sh, items, nitems);
+            else hipLaunchKernelGGL((k_stress_wl_lds<T, C, TY, NY>), g, block, 0, st, sh, items, nitems);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
     case 0: return sweep_tnd<T, C, TY, NY, 0, CP>(which, shots, items, nitems, grid, st);
     case 1: return sweep_tnd<T, C, TY, NY, 1, CP>(which, shots, items, nitems, grid, st);
     case 2: return sweep_tnd<T, C, TY, NY, 2, CP>(which, shots, items, nitems, grid, st);
@@ -1594,7 +1649,9 @@ static hipError_t sweep_tn(int which, int dm, bool comp, const void *shots, cons
 }
 
 // supported (tile rows, rows per thread): (4,1) (8,1) (8,2) (16,1) (16,2);
-// damping masks 0, 1, 2, 4, 7; comp: compensated time update (16-bit only)
+// damping masks 0, 1, 2, 4, 7, and DM_INTERIOR_LDS (interior boxes with the
+// staging rows loaded straight into LDS, FP32); comp: compensated time
+// update (16-bit only)
 template <class T, class C>
 hipError_t sweep_t(int which, int ty, int ny, int dm, bool comp, const void *shots, const WorkItem *items,
                    int nitems, int grid, hipStream_t st)

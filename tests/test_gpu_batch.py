@@ -64,6 +64,31 @@ def test_tile_shapes_are_bit_identical(sgfd_mod):
         assert np.array_equal(tr, tr0), tile
 
 
+def test_no_pml_lds_rows_are_bit_identical(sgfd_mod):
+    """A grid without PML: with separate list launches its interior boxes
+    load their staging rows straight into LDS (FP32); one stream runs the
+    fused launch, which stages through registers.  Every box shape, ragged
+    extents (150x77x61): the same bits."""
+    wl = _small_workload(sgfd_mod, 9, 150, 77, 61, pml=0)
+    _say("no PML, 150x77x61: streams 1 against 3, box shapes (4,1) (8,1) (8,2) (16,1) (16,2)")
+    ref = None
+    for tile in ((4, 1, 4, 1, 0), (8, 1, 4, 1, 0), (8, 2, 4, 1, 9), (16, 1, 4, 1, 20), (16, 2, 4, 1, 0)):
+        for n in (1, 3):
+            s = wl.solver()
+            try:
+                s.set_tile(*tile)
+                s.set_streams(n)
+                s.step(wl.nt)
+                s.synchronize()
+                out = (s.stations(), s.fields())
+            finally:
+                s.close()
+            if ref is None:
+                ref = out
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (tile, n)
+            assert np.isfinite(out[1]).all()
+
+
 def test_stream_counts_are_bit_identical(sgfd_mod):
     wl = _small_workload(sgfd_mod, 5, 150, 77, 61)
     _say("streams 1..4, 150x77x61, eager and graph")
