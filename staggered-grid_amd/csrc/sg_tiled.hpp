@@ -336,8 +336,9 @@ __device__ __forceinline__ void gst<bf16>(bf16 *p, unsigned off, bf16 v)
 // 512x512x384 P20 and 256^3 P16 unchanged (29.45 / 29.42, 24.26 / 24.27)
 // while the interior lists fetch 8-9 % more (more blocks share each L2);
 // with the PML lists as well it is slower (512x512x384: 28.96 with the x
-// faces, 28.82 with every list).  Off by default; worth turning on for grids
-// without PML.
+// faces, 28.82 with every list).  Off by default: the solver runs the
+// interior kernels of this form (k_velocity_wl_lds, k_stress_wl_lds) for
+// grids without PML, where the interior list runs alone.
 #ifndef SG_LDSDMA
 #define SG_LDSDMA 0
 #endif
