@@ -1508,7 +1508,8 @@ __device__ __forceinline__ void str_tile(const Shot<T, C> &s, const WorkItem &w,
 // kernels: a grid-stride loop over a work-list (one or many shots)
 // ---------------------------------------------------------------------------
 // Register caps (minimum waves per SIMD) of the face instantiations (DM 1,
-// 2, 4: SG_PML_WAVES; 16-bit storage: SG_PML_WAVES16) and of the interior
+// 2, 4: SG_PML_WAVES; 16-bit storage: SG_PML_WAVES16, velocity faces
+// SG_PML_WAVES16_V) and of the interior
 // ones (SG_INT_WAVES; 16-bit storage: SG_INT_WAVES16); edges and corners
 // are never capped
 #ifndef SG_PML_WAVES
@@ -1525,6 +1526,12 @@ __device__ __forceinline__ void str_tile(const Shot<T, C> &s, const WorkItem &w,
 // the interior velocity sweep at 4 (128 VGPRs, a 12-byte spill)
 #ifndef SG_PML_WAVES16
 #define SG_PML_WAVES16 3
+#endif
+// 16-bit velocity faces uncapped since the use-site conversion (512x512x384
+// FP16, 400 steps: velocity sweep 1.1645 -> 1.1356 ms uncapped, while the
+// stress sweep wants its faces at 3 waves: 1.5455 against 1.5764 ms)
+#ifndef SG_PML_WAVES16_V
+#define SG_PML_WAVES16_V 1
 #endif
 #ifndef SG_INT_WAVES
 #define SG_INT_WAVES 1
@@ -1543,8 +1550,9 @@ struct WavesCap {
     static constexpr bool F32 = sizeof(T) == 4, H = sizeof(T) == 2 && !CP;
     static constexpr int i16 = W == 0 ? SG_INT_WAVES16_V : SG_INT_WAVES16;
     static constexpr int f32 = W == 0 ? SG_PML_WAVES_V : SG_PML_WAVES;
+    static constexpr int h16 = W == 0 ? SG_PML_WAVES16_V : SG_PML_WAVES16;
     static constexpr int v = DM == 7 ? 1
-                             : DM   ? (F32 ? f32 : (H ? SG_PML_WAVES16 : 1))
+                             : DM   ? (F32 ? f32 : (H ? h16 : 1))
                                     : (NY == 1 ? (F32 ? SG_INT_WAVES : (H ? i16 : 1)) : 1);
 };
 
