@@ -1515,10 +1515,13 @@ __device__ __forceinline__ void str_tile(const Shot<T, C> &s, const WorkItem &w,
 #ifndef SG_PML_WAVES
 #define SG_PML_WAVES 1
 #endif
-// FP32 velocity faces at 3 waves (x face 170 -> 168 VGPRs): 512x512x384
-// 27.70 -> 27.79, 256^3 20.55 -> 20.91 Gpt/s; the stress faces lose with it
+// FP32 velocity faces: capped at 3 waves they gained before the use-site
+// conversion (512x512x384 27.70 -> 27.79, 256^3 20.55 -> 20.91 Gpt/s); since
+// it they run faster uncapped (velocity sweep 1.5069 -> 1.4986 ms, 256^3
+// 24.22 -> 24.45 Gpt/s); the stress faces lose with a cap (1.895 -> 1.908 ms
+// at 3 waves)
 #ifndef SG_PML_WAVES_V
-#define SG_PML_WAVES_V 3
+#define SG_PML_WAVES_V 1
 #endif
 // 16-bit storage, measured at 512x512x384 and 500^3 (FP16 31.3 -> 32.0 and
 // 37.7 -> 39.4 Gpt/s, BF16 30.5 -> 31.5; FP32 unaffected): faces at 3 waves
