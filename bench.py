@@ -175,6 +175,44 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
+def host_buffers(args):
+    """The whole job with the host copies the boundary makes: the model
+    uploaded from host arrays (sg_set_medium, host Courant check included),
+    the timed steps, the station traces and all nine final fields copied
+    back; each phase timed with a synchronisation.  Reported beside the
+    HBM-resident rate; never the bench value."""
+    import sgfd
+    from sgfd import configs
+
+    wl = configs.BUILDERS[args.workload]()
+    K = min(args.steps, wl.nt)
+    s = wl.solver()  # the first upload and a few steps warm the path
+    try:
+        s.step(min(10, K))
+        s.synchronize()
+        s.reset()
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.set_medium(wl.b, wl.lam, wl.mu)
+        s.synchronize()
+        t1 = time.perf_counter()
+        s.step(K)
+        s.synchronize()
+        t2 = time.perf_counter()
+        tr = s.stations()
+        t3 = time.perf_counter()
+        F = s.fields()
+        t4 = time.perf_counter()
+    finally:
+        s.close()
+    pts = wl.points * K
+    print(json.dumps({"metric": "Gpt-updates/s with host copies", "workload": wl.description, "steps": K,
+                      "upload_s": t1 - t0, "steps_s": t2 - t1, "stations_s": t3 - t2, "fields_s": t4 - t3,
+                      "model_bytes": 3 * wl.b.nbytes, "station_bytes": int(tr.nbytes), "field_bytes": int(F.nbytes),
+                      "gpt_resident": pts / (t2 - t1) / 1e9, "gpt_with_model_and_stations": pts / (t3 - t0) / 1e9,
+                      "gpt_with_final_fields": pts / (t4 - t0) / 1e9}))
+
+
 def bench_medium(args):
     """The seeded random-medium builder (the reference's spectral method,
     randomMedium/RandomMedia_3_d.m, with its own spectrum and parameters) on
@@ -320,6 +358,8 @@ def main():
                     help="batched32: also time the shots one solver at a time")
     ap.add_argument("--quick-study", action="store_true",
                     help="precision study: only FP64, FP32, FP16 and the FP16 error-split rows")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="the whole job with the model upload and the station and field downloads, FP32")
     ap.add_argument("--precision-study", action="store_true",
                     help="every storage precision and 16-bit mode over the full workload: speed and seismogram "
                          "error against FP64")
@@ -342,6 +382,9 @@ def main():
         return
     if args.workload == "medium512":
         bench_medium(args)
+        return
+    if args.host_buffers:
+        host_buffers(args)
         return
     if args.precision_study:
         precision_study(args)
